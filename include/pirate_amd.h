@@ -1,0 +1,155 @@
+/*
+ * pirate_amd.h — C-ABI of the MI355X (gfx950) dedispersion chain.
+ *
+ * Plain C: opaque handles, plain pointers and sizes; HIP streams are passed as void*
+ * (a hipStream_t, or NULL for the null stream). Every function returns 0 on success and a
+ * nonzero status on failure, with the message available from pdd_last_error() (thread-local).
+ * A dedisperser that has failed (or was stopped) latches the first error: every later call
+ * on it fails with that message (reference "stoppable" semantics,
+ * src_lib/GpuDedisperser.cpp:361-372, notes/stoppable_class.md).
+ *
+ * Each entry point names the reference interface it replaces. The reference exposes these
+ * through pybind11 (src_pybind11/pirate_pybind11.cpp); this library exposes them as C, and
+ * the Python package pirate_amd binds them with ctypes.
+ */
+
+#ifndef PIRATE_AMD_H
+#define PIRATE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdd_config pdd_config;
+typedef struct pdd_plan pdd_plan;
+typedef struct pdd_dedisperser pdd_dedisperser;
+
+/* Last error message of the calling thread ("" if none). */
+const char *pdd_last_error(void);
+
+/* ABI version of this library. */
+int pdd_abi_version(void);
+
+/* ---- DedispersionConfig (include/pirate/DedispersionConfig.hpp:21-260) ---- */
+
+/* DedispersionConfig::from_yaml(filename), src_lib/DedispersionConfig.cpp:643-717 (file or text). */
+int pdd_config_from_yaml_file(const char *filename, pdd_config **out);
+int pdd_config_from_yaml_string(const char *yaml_text, pdd_config **out);
+void pdd_config_destroy(pdd_config *config);
+
+/* DedispersionConfig::to_yaml_string(verbose), DedispersionConfig.cpp:635-640. Writes up to
+ * buflen bytes (NUL-terminated) and the required size (including NUL) to *needed. */
+int pdd_config_to_yaml(const pdd_config *config, int verbose, char *buf, size_t buflen, size_t *needed);
+
+/* DedispersionConfig::make_channel_map(), DedispersionConfig.cpp:238-250; out has 2^R+1 doubles. */
+int pdd_config_make_channel_map(const pdd_config *config, double *out, size_t n);
+
+/* DedispersionConfig::{frequency_to_index, index_to_frequency, delay_to_frequency,
+ * frequency_to_delay, dm_per_unit_delay, max_dm_of_all_trees}, DedispersionConfig.cpp:52-232.
+ * 'what' names the function; x is its argument (ignored for the last two). */
+int pdd_config_eval(const pdd_config *config, const char *what, double x, double *out);
+
+/* DedispersionConfig::test(), DedispersionConfig.cpp:275-330. */
+int pdd_config_test(const pdd_config *config);
+
+/* ---- DedispersionPlan (include/pirate/DedispersionPlan.hpp) ---- */
+
+/* DedispersionPlan(config, Params{gpu_runnable}), src_lib/DedispersionPlan.cpp:29-365. */
+int pdd_plan_create(const pdd_config *config, int gpu_runnable, pdd_plan **out);
+void pdd_plan_destroy(pdd_plan *plan);
+
+/* DedispersionPlan::to_yaml_string(verbose), DedispersionPlan.cpp:664-920. */
+int pdd_plan_to_yaml(const pdd_plan *plan, int verbose, char *buf, size_t buflen, size_t *needed);
+
+/* DedispersionTree fields, include/pirate/DedispersionTree.hpp:19-70. */
+typedef struct {
+    int64_t primary_tree_index, early_trigger_level, amb_rank, dd_rank, nt_ds;
+    int64_t max_width, dm_downsampling, time_downsampling, wt_dm_downsampling, wt_time_downsampling;
+    int64_t Dcore, nprofiles, ndm_out, ndm_wt, nt_out, nt_wt;
+    int64_t nsubbands_N, nmultiplets_M, pf_rank;
+    double dm_min, dm_max, trigger_frequency;
+} pdd_tree_info;
+
+int pdd_plan_ntrees(const pdd_plan *plan, int64_t *ntrees);
+int pdd_plan_tree_info(const pdd_plan *plan, int64_t itree, pdd_tree_info *out);
+
+/* DedispersionPlan::decode_argmax(), DedispersionPlan.cpp:481-547.
+ * out5 = {fmin, fmax, tlo, thi, p}. */
+int pdd_plan_decode_argmax(const pdd_plan *plan, uint32_t token, int64_t itree, int64_t idm_coarse,
+                           int64_t itime_coarse, int64_t *out5);
+
+/* DedispersionPlan::decode_argmax2(), DedispersionPlan.cpp:551-615.
+ * out5 = {freq_lo_MHz, freq_hi_MHz, dm, timestamp_samp, width_samp}. */
+int pdd_plan_decode_argmax2(const pdd_plan *plan, int64_t itree, int64_t fmin, int64_t fmax,
+                            int64_t tlo, int64_t thi, int64_t p, double *out5);
+
+/* DedispersionPlan::compute_steady_state_it0(itree), DedispersionPlan.cpp:618-661. */
+int pdd_plan_steady_state_it0(const pdd_plan *plan, int64_t itree, int64_t *out, size_t n);
+
+/* ---- GpuDedisperser (include/pirate/Dedisperser.hpp:45-392) ---- */
+
+/* GpuDedisperser::Params, Dedisperser.hpp:47-86 (stream_pool replaced by an internal
+ * compute stream; cuda_device_id -> device). Zero means "default" for the counts. */
+typedef struct {
+    int64_t nbatches_out;
+    int64_t nbatches_wt;
+    int64_t num_consumers;
+    int32_t device;
+    int32_t synchronous;
+    int64_t initial_chunk;
+} pdd_dedisperser_params;
+
+/* GpuDedisperser::create(params), src_lib/GpuDedisperser.cpp:90-372. */
+int pdd_dedisperser_create(const pdd_plan *plan, const pdd_dedisperser_params *params, pdd_dedisperser **out);
+void pdd_dedisperser_destroy(pdd_dedisperser *dd);
+
+/* GpuDedisperser::allocate(gpu_allocator, host_allocator), GpuDedisperser.cpp:375-467
+ * (this library allocates its own device memory). */
+int pdd_dedisperser_allocate(pdd_dedisperser *dd);
+
+/* Bytes of device memory held by the dedisperser. */
+int pdd_dedisperser_gpu_bytes(const pdd_dedisperser *dd, int64_t *nbytes);
+
+/* GpuDedisperser::acquire_input(seq_id, stream), GpuDedisperser.cpp:742-823.
+ * *slot receives the device address of a contiguous (beams_per_batch, nfreq, nt_in) array
+ * of the config dtype, valid until the matching release. */
+int pdd_acquire_input(pdd_dedisperser *dd, int64_t seq_id, void *stream, void **slot);
+
+/* GpuDedisperser::release_input_and_launch_dd_kernels(seq_id, stream), GpuDedisperser.cpp:826-855. */
+int pdd_release_input_and_launch(pdd_dedisperser *dd, int64_t seq_id, void *stream);
+
+/* GpuDedisperser::acquire_output(consumer, seq_id, stream, sync), GpuDedisperser.cpp:858-935.
+ * out_max[itree] / out_argmax[itree] (arrays of length ntrees) receive device addresses of
+ * contiguous (beams_per_batch, ndm_out, nt_out) arrays (config dtype / uint32). */
+int pdd_acquire_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream, int sync,
+                       void **out_max, uint32_t **out_argmax, int64_t ntrees);
+
+/* GpuDedisperser::release_output(consumer, seq_id, stream), GpuDedisperser.cpp:938-984. */
+int pdd_release_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream);
+
+/* GpuDedisperser::fill_all_weights(itree, weights), GpuDedisperser.cpp:1257-1279.
+ * host_weights: float32 (nbatches_wt, beams_per_batch, ndm_wt, nt_wt, nprofiles, N), >= 0. */
+int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *host_weights, int64_t nelts);
+
+/* GpuDedisperser::stop(e), GpuDedisperser.cpp:520-577. */
+int pdd_stop(pdd_dedisperser *dd, const char *reason);
+
+/* Kernel timing (reference counterpart: the per-kernel KernelTimer loops and ResourceTracker
+ * rates of GpuDedisperser::time, src_lib/GpuDedisperser.cpp:1573-1690). When enabled, HIP
+ * events bracket every kernel launch on the compute stream; the report holds one line
+ * "name launches total_ms" per kernel (gridding, lds, stage1_p<p>, stage2_t<i>, pf_t<i>). */
+int pdd_kernel_timing(pdd_dedisperser *dd, int enable);
+int pdd_kernel_timing_report(pdd_dedisperser *dd, char *buf, size_t buflen, size_t *needed);
+
+/* Test hook (no reference counterpart): copies an internal device buffer to the host.
+ * name: "grid", "lds<p>", "sb<itree>", "ring<p>". With host == NULL only *nbytes is set. */
+int pdd_debug_copy(pdd_dedisperser *dd, const char *name, void *host, size_t maxbytes, size_t *nbytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIRATE_AMD_H */

@@ -1,0 +1,12 @@
+"""pirate_amd: the pirate FRB-search GPU dedispersion chain, rebuilt for AMD MI355X (gfx950).
+
+Python surface (mirrors pirate_frb's DedispersionConfig / DedispersionPlan / GpuDedisperser)
+over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library with
+``python -m pirate_amd.build``.
+"""
+
+from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
+from .dedisperser import GpuDedisperser, GpuDedisperserOutputs
+
+__all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
+           'GpuDedisperser', 'GpuDedisperserOutputs']

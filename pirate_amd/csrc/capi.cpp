@@ -1,0 +1,297 @@
+// extern "C" entry points of libpirate_amd.so (declared in include/pirate_amd.h).
+
+#include "../../include/pirate_amd.h"
+#include "host/common.hpp"
+#include "host/config.hpp"
+#include "host/plan.hpp"
+#include "gpu/dedisperser.hpp"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+using namespace pirate_amd;
+
+struct pdd_config { DedispersionConfig c; };
+struct pdd_plan { std::shared_ptr<DedispersionPlan> p; };
+struct pdd_dedisperser { std::unique_ptr<GpuDedisperser> d; };
+
+static thread_local std::string g_last_error;
+
+static int fail(const std::string &msg)
+{
+    g_last_error = msg;
+    return -1;
+}
+
+#define PDD_TRY(body)                                                     \
+    try {                                                                 \
+        body;                                                             \
+        return 0;                                                         \
+    }                                                                     \
+    catch (std::exception &e) { return fail(e.what()); }                  \
+    catch (...) { return fail("unknown C++ exception"); }
+
+// Dedisperser entry points: a throw stops the dedisperser (first error latches).
+#define PDD_TRY_DD(dd, body)                                              \
+    try {                                                                 \
+        if (!(dd) || !(dd)->d) return fail("null dedisperser handle");    \
+        body;                                                             \
+        return 0;                                                         \
+    }                                                                     \
+    catch (std::exception &e) { (dd)->d->stop(e.what()); return fail(e.what()); } \
+    catch (...) { (dd)->d->stop("unknown C++ exception"); return fail("unknown C++ exception"); }
+
+static int copy_string(const std::string &s, char *buf, size_t buflen, size_t *needed)
+{
+    if (needed)
+        *needed = s.size() + 1;
+    if (buf && buflen > 0) {
+        size_t n = std::min(buflen - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return 0;
+}
+
+extern "C" {
+
+const char *pdd_last_error(void) { return g_last_error.c_str(); }
+int pdd_abi_version(void) { return 1; }
+
+int pdd_config_from_yaml_file(const char *filename, pdd_config **out)
+{
+    PDD_TRY({
+        PDD_CHECK(filename && out, "null argument");
+        *out = new pdd_config{DedispersionConfig::from_yaml_file(filename)};
+    });
+}
+
+int pdd_config_from_yaml_string(const char *text, pdd_config **out)
+{
+    PDD_TRY({
+        PDD_CHECK(text && out, "null argument");
+        *out = new pdd_config{DedispersionConfig::from_yaml_string(text)};
+    });
+}
+
+void pdd_config_destroy(pdd_config *c) { delete c; }
+
+int pdd_config_to_yaml(const pdd_config *c, int verbose, char *buf, size_t buflen, size_t *needed)
+{
+    PDD_TRY({
+        PDD_CHECK(c, "null config");
+        copy_string(c->c.to_yaml_string(verbose != 0), buf, buflen, needed);
+    });
+}
+
+int pdd_config_make_channel_map(const pdd_config *c, double *out, size_t n)
+{
+    PDD_TRY({
+        PDD_CHECK(c && out, "null argument");
+        std::vector<double> m = c->c.make_channel_map();
+        PDD_CHECK(n >= m.size(), "pdd_config_make_channel_map: output too small (need " << m.size() << ")");
+        std::copy(m.begin(), m.end(), out);
+    });
+}
+
+int pdd_config_eval(const pdd_config *c, const char *what, double x, double *out)
+{
+    PDD_TRY({
+        PDD_CHECK(c && what && out, "null argument");
+        std::string w(what);
+        if (w == "frequency_to_index") *out = c->c.frequency_to_index(x);
+        else if (w == "index_to_frequency") *out = c->c.index_to_frequency(x);
+        else if (w == "delay_to_frequency") *out = c->c.delay_to_frequency(x);
+        else if (w == "frequency_to_delay") *out = c->c.frequency_to_delay(x);
+        else if (w == "dm_per_unit_delay") *out = c->c.dm_per_unit_delay();
+        else if (w == "max_dm_of_all_trees") *out = c->c.max_dm_of_all_trees();
+        else throw pdd_error("pdd_config_eval: unknown function '" + w + "'");
+    });
+}
+
+int pdd_config_test(const pdd_config *c)
+{
+    PDD_TRY({ PDD_CHECK(c, "null config"); c->c.test(); });
+}
+
+int pdd_plan_create(const pdd_config *c, int gpu_runnable, pdd_plan **out)
+{
+    PDD_TRY({
+        PDD_CHECK(c && out, "null argument");
+        *out = new pdd_plan{std::make_shared<DedispersionPlan>(c->c, gpu_runnable != 0)};
+    });
+}
+
+void pdd_plan_destroy(pdd_plan *p) { delete p; }
+
+int pdd_plan_to_yaml(const pdd_plan *p, int verbose, char *buf, size_t buflen, size_t *needed)
+{
+    PDD_TRY({
+        PDD_CHECK(p, "null plan");
+        copy_string(p->p->to_yaml_string(verbose != 0), buf, buflen, needed);
+    });
+}
+
+int pdd_plan_ntrees(const pdd_plan *p, int64_t *n)
+{
+    PDD_TRY({ PDD_CHECK(p && n, "null argument"); *n = p->p->ntrees; });
+}
+
+int pdd_plan_tree_info(const pdd_plan *p, int64_t itree, pdd_tree_info *o)
+{
+    PDD_TRY({
+        PDD_CHECK(p && o, "null argument");
+        PDD_CHECK(itree >= 0 && itree < p->p->ntrees, "itree out of range");
+        const DedispersionTree &t = p->p->trees[itree];
+        o->primary_tree_index = t.primary_tree_index;
+        o->early_trigger_level = t.early_trigger_level;
+        o->amb_rank = t.amb_rank;
+        o->dd_rank = t.dd_rank;
+        o->nt_ds = t.nt_ds;
+        o->max_width = t.pf.max_width;
+        o->dm_downsampling = t.pf.dm_downsampling;
+        o->time_downsampling = t.pf.time_downsampling;
+        o->wt_dm_downsampling = t.pf.wt_dm_downsampling;
+        o->wt_time_downsampling = t.pf.wt_time_downsampling;
+        o->Dcore = t.Dcore;
+        o->nprofiles = t.nprofiles;
+        o->ndm_out = t.ndm_out;
+        o->ndm_wt = t.ndm_wt;
+        o->nt_out = t.nt_out;
+        o->nt_wt = t.nt_wt;
+        o->nsubbands_N = t.frequency_subbands.N;
+        o->nmultiplets_M = t.frequency_subbands.M;
+        o->pf_rank = t.frequency_subbands.pf_rank;
+        o->dm_min = t.dm_min;
+        o->dm_max = t.dm_max;
+        o->trigger_frequency = t.trigger_frequency;
+    });
+}
+
+int pdd_plan_decode_argmax(const pdd_plan *p, uint32_t token, int64_t itree, int64_t idm, int64_t it, int64_t *out5)
+{
+    PDD_TRY({
+        PDD_CHECK(p && out5, "null argument");
+        long v[5];
+        p->p->decode_argmax(token, itree, idm, it, v[0], v[1], v[2], v[3], v[4]);
+        for (int i = 0; i < 5; i++) out5[i] = v[i];
+    });
+}
+
+int pdd_plan_decode_argmax2(const pdd_plan *p, int64_t itree, int64_t fmin, int64_t fmax, int64_t tlo, int64_t thi,
+                            int64_t pp, double *out5)
+{
+    PDD_TRY({
+        PDD_CHECK(p && out5, "null argument");
+        p->p->decode_argmax2(itree, fmin, fmax, tlo, thi, pp, out5[0], out5[1], out5[2], out5[3], out5[4]);
+    });
+}
+
+int pdd_plan_steady_state_it0(const pdd_plan *p, int64_t itree, int64_t *out, size_t n)
+{
+    PDD_TRY({
+        PDD_CHECK(p && out, "null argument");
+        std::vector<long> v = p->p->compute_steady_state_it0(itree);
+        PDD_CHECK(n >= v.size(), "pdd_plan_steady_state_it0: output too small");
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+int pdd_dedisperser_create(const pdd_plan *p, const pdd_dedisperser_params *prm, pdd_dedisperser **out)
+{
+    PDD_TRY({
+        PDD_CHECK(p && out, "null argument");
+        GpuDedisperserParams gp;
+        if (prm) {
+            gp.nbatches_out = prm->nbatches_out;
+            gp.nbatches_wt = prm->nbatches_wt;
+            gp.num_consumers = prm->num_consumers ? prm->num_consumers : 1;
+            gp.device = prm->device;
+            gp.synchronous = prm->synchronous != 0;
+            gp.initial_chunk = prm->initial_chunk;
+        }
+        *out = new pdd_dedisperser{std::make_unique<GpuDedisperser>(p->p, gp)};
+    });
+}
+
+void pdd_dedisperser_destroy(pdd_dedisperser *dd) { delete dd; }
+
+int pdd_dedisperser_allocate(pdd_dedisperser *dd) { PDD_TRY_DD(dd, dd->d->allocate()); }
+
+int pdd_dedisperser_gpu_bytes(const pdd_dedisperser *dd, int64_t *nbytes)
+{
+    PDD_TRY({ PDD_CHECK(dd && nbytes, "null argument"); *nbytes = int64_t(dd->d->gpu_bytes_allocated); });
+}
+
+int pdd_acquire_input(pdd_dedisperser *dd, int64_t seq_id, void *stream, void **slot)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(slot, "null slot pointer");
+        *slot = dd->d->acquire_input(seq_id, (hipStream_t) stream);
+    });
+}
+
+int pdd_release_input_and_launch(pdd_dedisperser *dd, int64_t seq_id, void *stream)
+{
+    PDD_TRY_DD(dd, dd->d->release_input_and_launch(seq_id, (hipStream_t) stream));
+}
+
+int pdd_acquire_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream, int sync,
+                       void **out_max, uint32_t **out_argmax, int64_t ntrees)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(out_max && out_argmax, "null output arrays");
+        PDD_CHECK(ntrees == dd->d->ntrees, "pdd_acquire_output: ntrees mismatch");
+        std::vector<void *> om;
+        std::vector<unsigned int *> oa;
+        dd->d->acquire_output(consumer, seq_id, (hipStream_t) stream, sync != 0, om, oa);
+        for (int64_t i = 0; i < ntrees; i++) {
+            out_max[i] = om[i];
+            out_argmax[i] = oa[i];
+        }
+    });
+}
+
+int pdd_release_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream)
+{
+    PDD_TRY_DD(dd, dd->d->release_output(consumer, seq_id, (hipStream_t) stream));
+}
+
+int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *w, int64_t nelts)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(w, "null weights");
+        dd->d->fill_all_weights(itree, w, nelts);
+    });
+}
+
+int pdd_stop(pdd_dedisperser *dd, const char *reason)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d, "null dedisperser handle");
+        dd->d->stop(reason ? reason : "");
+    });
+}
+
+int pdd_kernel_timing(pdd_dedisperser *dd, int enable)
+{
+    PDD_TRY_DD(dd, dd->d->set_kernel_timing(enable != 0));
+}
+
+int pdd_kernel_timing_report(pdd_dedisperser *dd, char *buf, size_t buflen, size_t *needed)
+{
+    PDD_TRY_DD(dd, copy_string(dd->d->kernel_timing_report(), buf, buflen, needed));
+}
+
+int pdd_debug_copy(pdd_dedisperser *dd, const char *name, void *host, size_t maxbytes, size_t *nbytes)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d && name, "null argument");
+        size_t n = dd->d->debug_copy(name, host, maxbytes);
+        if (nbytes)
+            *nbytes = n;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,773 @@
+#include "dedisperser.hpp"
+#include "../host/common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pirate_amd {
+
+bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp);   // k_pf.hip
+
+#define HIP_CHECK(expr)                                                                   \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            throw pdd_error(std::string("HIP error: ") + hipGetErrorString(_e) + " in " + #expr); \
+    } while (0)
+
+static constexpr int kGpuMaxDD = 7;
+static constexpr long kLdsBytesPerCU = 160 * 1024;
+
+
+GpuDedisperser::GpuDedisperser(std::shared_ptr<DedispersionPlan> plan_, const GpuDedisperserParams &params_) :
+    plan(plan_), params(params_)
+{
+    PDD_CHECK(plan, "GpuDedisperser: null plan");
+    PDD_CHECK(plan->gpu_runnable, "GpuDedisperser: plan was built with gpu_runnable=false");
+    dtype = plan->dtype;
+    elt_bytes = dtype_nbits(dtype) / 8;
+    nfreq = plan->nfreq;
+    nt_in = plan->nt_in;
+    total_beams = plan->beams_per_gpu;
+    beams_per_batch = plan->beams_per_batch;
+    nbatches = total_beams / beams_per_batch;
+    nstreams = plan->num_active_batches;
+    ntrees = plan->ntrees;
+
+    nbatches_out = params.nbatches_out ? params.nbatches_out : nstreams;
+    nbatches_wt = params.nbatches_wt ? params.nbatches_wt : nbatches_out;
+    num_consumers = params.num_consumers;
+    PDD_CHECK(nbatches_out > 0 && nbatches_wt > 0, "GpuDedisperser: nbatches_out and nbatches_wt must be > 0");
+    PDD_CHECK(num_consumers >= 1, "GpuDedisperser: num_consumers must be >= 1");
+    PDD_CHECK(params.initial_chunk >= 0, "GpuDedisperser: initial_chunk must be >= 0");
+
+    curr_output_acquire.assign(num_consumers, 0);
+    curr_output_release.assign(num_consumers, 0);
+
+    build_tables();
+    validate_gpu_limits();
+}
+
+
+GpuDedisperser::~GpuDedisperser()
+{
+    if (compute_stream)
+        (void) hipStreamSynchronize(compute_stream);
+    auto freeit = [](DeviceArray &a) { if (a.ptr) (void) hipFree(a.ptr); a.ptr = nullptr; };
+    freeit(input_slots);
+    freeit(grid_buf);
+    freeit(lds_state);
+    freeit(grid_rows);
+    freeit(grid_w);
+    for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &stage2_lags, &sb, &d_main_emit, &d_stag,
+                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &s1_state, &s2_state, &weights, &out_max, &out_argmax})
+        for (auto &a : *v)
+            freeit(a);
+    for (auto &t : timed_pending) {
+        timing_event_pool.push_back(t.start);
+        timing_event_pool.push_back(t.stop);
+    }
+    for (auto *v : {&ev_input_free, &ev_input_ready, &ev_output_ready, &ev_output_released, &timing_event_pool})
+        for (hipEvent_t e : *v)
+            (void) hipEventDestroy(e);
+    if (compute_stream)
+        (void) hipStreamDestroy(compute_stream);
+}
+
+
+// Subband-stream tables of each stage-2 tree. The loops restate the subband extraction
+// of ReferenceTree::dedisperse_2d (src_lib/ReferenceTree.cpp:142-287) for one tree
+// instance: level-0 copies, main-pair outputs (even subband index), staggered extra
+// butterflies (odd subband index), multiplet numbering, and the final lags
+// (ReferenceTree.cpp:42-54).
+void GpuDedisperser::build_tables()
+{
+    tables.resize(ntrees);
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        const FrequencySubbands &fs = t.frequency_subbands;
+        TreeTables &T = tables[it];
+        const int dd = t.dd_rank, pf_rank = int(fs.pf_rank);
+        const int NP = 1 << (dd - 1), M = int(fs.M);
+        const int pf_ndm = 1 << (dd - pf_rank);
+        T.M = M;
+        T.N = int(fs.N);
+        T.main_emit.assign(dd * NP, -1);
+        T.fl.assign(pf_ndm * M, 0);
+        long mcurr = 0;
+        int hoff = 0;
+        for (int r = 0; r <= dd; r++) {
+            const int ndm_in = 1 << r;
+            if (r == dd - pf_rank) {
+                const int pf_ns = int(fs.subband_counts[0]);
+                for (int dm_in = 0; dm_in < ndm_in; dm_in++)
+                    for (int pfs = 0; pfs < pf_ns; pfs++) {
+                        T.lvl0.push_back(pfs * ndm_in + int(bit_reverse(dm_in, r)));
+                        T.lvl0.push_back(dm_in * M + pfs);
+                    }
+                mcurr += pf_ns;
+            }
+            if (r >= dd)
+                break;
+            const int pf_level = r + pf_rank - dd + 1;
+            if (pf_level <= 0)
+                continue;
+            const int pf_ns = int(fs.subband_counts[pf_level]);
+            const int pf_nd2 = 1 << (pf_level - 1);
+            for (int pf_dm = 0; pf_dm < pf_ndm; pf_dm++)
+                for (int d2 = 0; d2 < pf_nd2; d2++) {
+                    const int dm_in = pf_dm * pf_nd2 + d2;
+                    const int dmb = int(bit_reverse(dm_in, r));
+                    for (int pfs = 0; pfs < pf_ns; pfs++) {
+                        const int m = int(mcurr) + pfs * 2 * pf_nd2 + 2 * d2;
+                        const int s0 = pf_dm * M + m;
+                        if ((pfs & 1) == 0) {
+                            const int p = (pfs / 2) * ndm_in + dmb;
+                            T.main_emit[r * NP + p] = s0;
+                        }
+                        else {
+                            int e[6] = {r, pfs * ndm_in + dmb, (pfs + 1) * ndm_in + dmb, dm_in, s0, hoff};
+                            T.stag.insert(T.stag.end(), e, e + 6);
+                            hoff += ndm_in;
+                        }
+                    }
+                }
+            mcurr += long(pf_ns) * 2 * pf_nd2;
+        }
+        PDD_CHECK(mcurr == M, "GpuDedisperser: internal error: multiplet count mismatch");
+        T.nstag = int(T.stag.size() / 6);
+        // Entries were generated pass by pass: record each pass's range.
+        T.stag_begin.assign(dd + 1, 0);
+        for (int r = 0, k = 0; r <= dd; r++) {
+            while (k < T.nstag && T.stag[6 * k] < r)
+                k++;
+            T.stag_begin[r] = k;
+        }
+        T.nlvl0 = int(T.lvl0.size() / 2);
+        T.stag_hist_nelts = hoff;
+
+        for (int pf_dm = 0; pf_dm < pf_ndm; pf_dm++)
+            for (int m = 0; m < M; m++) {
+                int v = int((pow2(pf_rank) - fs.m_to_fhi(m)) * pf_dm);
+                T.fl[pf_dm * M + m] = v;
+                T.flmax = std::max(T.flmax, v);
+            }
+        for (int m = 0; m < M; m++)
+            T.m_to_n.push_back(int(fs.m_to_n[m]));
+
+        T.tpad = int(std::max(2 * t.pf.max_width, 4L));
+        T.nlev = std::max(integer_log2(t.pf.max_width), 1);
+        T.rowlen = T.tpad + t.nt_ds + T.flmax;
+    }
+}
+
+
+void GpuDedisperser::validate_gpu_limits()
+{
+    const long R = plan->config.toplevel_tree_rank;
+    const long npri = plan->num_primary_trees;
+    PDD_CHECK(npri <= 7, "GpuDedisperser: at most 7 primary trees");
+    PDD_CHECK(nt_in % 4 == 0, "GpuDedisperser: nt_in must be a multiple of 4");
+    for (long p = 0; p < npri; p++) {
+        long Tp = nt_in >> p;
+        PDD_CHECK(Tp % 64 == 0, "GpuDedisperser: this build needs nt_in/2^p to be a multiple of 64 for every primary tree p (nt_in="
+                  << nt_in << ", p=" << p << ")");
+        PDD_CHECK(plan->stage1_dd_rank[p] <= kGpuMaxDD && plan->stage1_dd_rank[p] >= 1,
+                  "GpuDedisperser: stage-1 dd_rank " << plan->stage1_dd_rank[p] << " not supported by this build (max " << kGpuMaxDD << ")");
+        if (p > 0) {
+            long s = plan->stage1_dd_rank[1] + 1;
+            PDD_CHECK(pow2(s - 1) <= Tp, "GpuDedisperser: lagged-downsampling lag exceeds the chunk at p=" << p);
+        }
+    }
+    (void) R;
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        const TreeTables &T = tables[it];
+        PDD_CHECK(t.dd_rank <= kGpuMaxDD, "GpuDedisperser: stage-2 dd_rank " << t.dd_rank << " not supported by this build (max " << kGpuMaxDD << ")");
+        PDD_CHECK(stage2_lds_bytes(t.dd_rank, T.stag_hist_nelts) <= kLdsBytesPerCU,
+                  "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
+        std::vector<int> per_pass(t.dd_rank, 0);
+        for (int k = 0; k < T.nstag; k++)
+            per_pass[T.stag[6 * k]]++;
+        for (int c : per_pass)
+            PDD_CHECK(c <= 16, "GpuDedisperser: tree " << it << " has more than 16 staggered subband pairs in one pass (not supported by this build)");
+        PDD_CHECK(t.nt_ds >= T.tpad + T.flmax, "GpuDedisperser: tree " << it << ": chunk too short for peak-finder history + final lags");
+        PDD_CHECK(t.pf.time_downsampling <= 512, "GpuDedisperser: time_downsampling too large");
+        PDD_CHECK(pf_config_supported(T.M, T.tpad, T.nlev, int(t.Dcore), int(t.pf.time_downsampling), t.nt_ds),
+                  "GpuDedisperser: tree " << it << ": peak-finder shape not supported (M=" << T.M << ")");
+    }
+}
+
+
+DeviceArray GpuDedisperser::dalloc(size_t nbytes)
+{
+    DeviceArray a;
+    a.nbytes = nbytes;
+    if (nbytes == 0)
+        return a;
+    HIP_CHECK(hipMalloc(&a.ptr, nbytes));
+    HIP_CHECK(hipMemset(a.ptr, 0, nbytes));
+    gpu_bytes_allocated += nbytes;
+    return a;
+}
+
+
+void GpuDedisperser::allocate()
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    PDD_CHECK(!is_allocated, "GpuDedisperser::allocate() called twice");
+    HIP_CHECK(hipSetDevice(params.device));
+    HIP_CHECK(hipStreamCreateWithFlags(&compute_stream, hipStreamNonBlocking));
+
+    const long B = beams_per_batch, R = plan->config.toplevel_tree_rank, npri = plan->num_primary_trees;
+    const long N = pow2(R);
+
+    auto up_int = [&](const std::vector<int> &v) {
+        DeviceArray a = dalloc(std::max<size_t>(v.size(), 1) * sizeof(int));
+        if (!v.empty())
+            HIP_CHECK(hipMemcpy(a.ptr, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+        return a;
+    };
+    auto up_long = [&](const std::vector<long> &v) {
+        DeviceArray a = dalloc(std::max<size_t>(v.size(), 1) * sizeof(long));
+        if (!v.empty())
+            HIP_CHECK(hipMemcpy(a.ptr, v.data(), v.size() * sizeof(long), hipMemcpyHostToDevice));
+        return a;
+    };
+
+    input_slots = dalloc(size_t(nstreams) * B * nfreq * nt_in * elt_bytes);
+    grid_buf = dalloc(size_t(B) * N * nt_in * elt_bytes);
+
+    // Gridding CSR: per tree channel n, ascending input channels f in [if0, if1) with
+    // overlap weights (reference: src_lib/TreeGriddingKernel.cu:77-100).
+    {
+        std::vector<double> cmap = plan->config.make_channel_map();
+        std::vector<int> rows(3 * N);
+        std::vector<float> w;
+        for (long n = 0; n < N; n++) {
+            double f0 = cmap[n + 1], f1 = cmap[n];
+            long if0 = std::max(long(f0), 0L);
+            long if1 = std::min(long(f1) + 1, nfreq);
+            rows[n] = int(if0);
+            rows[N + n] = int(std::max(if1 - if0, 0L));
+            rows[2 * N + n] = int(w.size());
+            for (long f = if0; f < if1; f++) {
+                double flo = std::max(f0, double(f)), fhi = std::min(f1, double(f) + 1);
+                w.push_back(float(std::max(fhi - flo, 0.0)));
+            }
+        }
+        grid_rows = up_int(rows);
+        grid_w = dalloc(std::max<size_t>(w.size(), 1) * sizeof(float));
+        HIP_CHECK(hipMemcpy(grid_w.ptr, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+
+    lds_out.resize(npri);
+    for (long p = 1; p < npri; p++)
+        lds_out[p] = dalloc(size_t(B) * (N / 2) * (nt_in >> p) * elt_bytes);
+    if (npri > 1) {
+        long s = plan->stage1_dd_rank[1] + 1;
+        lds_state_bstride = lds_state_nelts(int(R), int(npri), int(s));
+        lds_state = dalloc(size_t(total_beams) * lds_state_bstride * sizeof(float));
+    }
+
+    // Lag ring: per primary tree p and stage-1 output row (a1, j), a circular row whose
+    // capacity covers the largest stage-2 lag of that row plus one chunk.
+    rings.resize(npri);
+    ring_row_off.resize(npri);
+    ring_row_cap.resize(npri);
+    ring_bstride.resize(npri);
+    s1_state.resize(npri);
+    s1_state_bstride.resize(npri);
+    for (long p = 0; p < npri; p++) {
+        const int dd1 = int(plan->stage1_dd_rank[p]), amb1 = int(plan->stage1_amb_rank[p]);
+        const long Tp = nt_in >> p;
+        const long nrows = pow2(dd1 + amb1);
+        std::vector<long> off(nrows), cap(nrows);
+        long tot = 0;
+        for (long a1 = 0; a1 < pow2(amb1); a1++)
+            for (long j = 0; j < pow2(dd1); j++) {
+                long maxlag = 0;
+                for (long it = 0; it < ntrees; it++) {
+                    const DedispersionTree &t = plan->trees[it];
+                    if (t.primary_tree_index != p || a1 >= pow2(t.dd_rank))
+                        continue;
+                    maxlag = std::max(maxlag, rb_lag(a1, j, t.amb_rank, t.dd_rank, p > 0));
+                }
+                long r = a1 * pow2(dd1) + j;
+                cap[r] = align_up(maxlag + Tp, 64);
+                off[r] = tot;
+                tot += cap[r];
+            }
+        ring_bstride[p] = align_up(tot, 64);
+        rings[p] = dalloc(size_t(total_beams) * ring_bstride[p] * elt_bytes);
+        ring_row_off[p] = up_long(off);
+        ring_row_cap[p] = up_long(cap);
+        s1_state_bstride[p] = pow2(amb1) * stage1_state_nelts_per_tree(dd1);
+        s1_state[p] = dalloc(size_t(total_beams) * s1_state_bstride[p] * sizeof(float));
+    }
+
+    stage2_lags.resize(ntrees);
+    sb.resize(ntrees);
+    d_main_emit.resize(ntrees);
+    d_stag.resize(ntrees);
+    d_lvl0.resize(ntrees);
+    d_fl.resize(ntrees);
+    d_m_to_n.resize(ntrees);
+    d_stag_begin.resize(ntrees);
+    s2_state.resize(ntrees);
+    s2_state_bstride.resize(ntrees);
+    weights.resize(ntrees);
+    out_max.resize(ntrees);
+    out_argmax.resize(ntrees);
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        const TreeTables &T = tables[it];
+        std::vector<int> lags(pow2(t.dd_rank + t.amb_rank));
+        for (long fc = 0; fc < pow2(t.dd_rank); fc++)
+            for (long j = 0; j < pow2(t.amb_rank); j++)
+                lags[fc * pow2(t.amb_rank) + j] = int(rb_lag(fc, j, t.amb_rank, t.dd_rank, t.primary_tree_index > 0));
+        stage2_lags[it] = up_int(lags);
+        // Peak-finder input rows hold per-beam history (and final-lag "future" samples)
+        // across chunks: one set of rows per beam of the GPU, not per batch.
+        sb[it] = dalloc(size_t(total_beams) * t.ndm_out * T.M * T.rowlen * elt_bytes);
+        d_main_emit[it] = up_int(T.main_emit);
+        d_stag[it] = up_int(T.stag);
+        d_lvl0[it] = up_int(T.lvl0);
+        d_fl[it] = up_int(T.fl);
+        d_m_to_n[it] = up_int(T.m_to_n);
+        d_stag_begin[it] = up_int(T.stag_begin);
+        s2_state_bstride[it] = pow2(t.amb_rank) * stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts);
+        s2_state[it] = dalloc(size_t(total_beams) * s2_state_bstride[it] * sizeof(float));
+        weights[it] = dalloc(size_t(nbatches_wt) * B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N * sizeof(float));
+        out_max[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * elt_bytes);
+        out_argmax[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * sizeof(unsigned int));
+    }
+
+    auto mkev = [](std::vector<hipEvent_t> &v, long n) {
+        v.resize(n);
+        for (auto &e : v)
+            HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    };
+    mkev(ev_input_free, nstreams);
+    mkev(ev_input_ready, nstreams);
+    mkev(ev_output_ready, nbatches_out);
+    mkev(ev_output_released, nbatches_out * num_consumers);
+    // Every slot starts out "free": record once so that first waits return immediately.
+    for (auto &e : ev_input_free) HIP_CHECK(hipEventRecord(e, compute_stream));
+    for (auto &e : ev_output_released) HIP_CHECK(hipEventRecord(e, compute_stream));
+    HIP_CHECK(hipStreamSynchronize(compute_stream));
+
+    is_allocated = true;
+}
+
+
+void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t st)
+{
+    const long B = beams_per_batch, R = plan->config.toplevel_tree_rank, npri = plan->num_primary_trees;
+    const long N = pow2(R);
+    const int beam0 = int(ibatch * B);
+
+    {
+        GriddingArgs g;
+        g.in = input;
+        g.out = grid_buf.ptr;
+        g.in_bstride = nfreq * nt_in;
+        g.out_bstride = N * nt_in;
+        g.B = int(B);
+        g.nchan = int(N);
+        g.nfreq = int(nfreq);
+        g.ntime = int(nt_in);
+        g.row_f0 = (const int *) grid_rows.ptr;
+        g.row_cnt = g.row_f0 + N;
+        g.row_woff = g.row_f0 + 2 * N;
+        g.w = (const float *) grid_w.ptr;
+        hipEvent_t ev;
+        timed_begin("gridding", st, ev);
+        launch_gridding(dtype, g, st);
+        timed_end("gridding", st, ev);
+    }
+    if (npri > 1) {
+        LdsArgs l;
+        l.in = grid_buf.ptr;
+        l.in_bstride = N * nt_in;
+        for (long p = 1; p < npri; p++) {
+            l.out[p] = lds_out[p].ptr;
+            l.out_bstride[p] = (N / 2) * (nt_in >> p);
+        }
+        l.state = (float *) lds_state.ptr;
+        l.state_bstride = lds_state_bstride;
+        l.B = int(B);
+        l.R = int(R);
+        l.npri = int(npri);
+        l.ntime = int(nt_in);
+        l.s = int(plan->stage1_dd_rank[1] + 1);
+        l.beam0 = beam0;
+        hipEvent_t ev;
+        timed_begin("lds", st, ev);
+        launch_lds(dtype, l, st);
+        timed_end("lds", st, ev);
+    }
+    for (long p = 0; p < npri; p++) {
+        Stage1Args s;
+        const long Tp = nt_in >> p;
+        s.in = p ? lds_out[p].ptr : grid_buf.ptr;
+        s.in_bstride = p ? (N / 2) * Tp : N * nt_in;
+        s.ring.base = rings[p].ptr;
+        s.ring.row_off = (const long *) ring_row_off[p].ptr;
+        s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
+        s.ring.bstride = ring_bstride[p];
+        s.state = (float *) s1_state[p].ptr;
+        s.state_bstride = s1_state_bstride[p];
+        s.B = int(B);
+        s.amb = int(plan->stage1_amb_rank[p]);
+        s.dd = int(plan->stage1_dd_rank[p]);
+        s.Tp = int(Tp);
+        s.beam0 = beam0;
+        s.chunk = ichunk;
+        hipEvent_t ev;
+        const std::string nm = "stage1_p" + std::to_string(p);
+        timed_begin(nm, st, ev);
+        launch_stage1(dtype, s, st);
+        timed_end(nm, st, ev);
+    }
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        const TreeTables &T = tables[it];
+        const long p = t.primary_tree_index;
+        Stage2Args s;
+        s.ring.base = rings[p].ptr;
+        s.ring.row_off = (const long *) ring_row_off[p].ptr;
+        s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
+        s.ring.bstride = ring_bstride[p];
+        s.row_lag = (const int *) stage2_lags[it].ptr;
+        s.sb = sb[it].ptr;
+        s.sb_bstride = t.ndm_out * T.M * T.rowlen;
+        s.rowlen = T.rowlen;
+        s.tpad = T.tpad;
+        s.main_emit = (const int *) d_main_emit[it].ptr;
+        s.stag = (const int *) d_stag[it].ptr;
+        s.lvl0 = (const int *) d_lvl0[it].ptr;
+        s.fl = (const int *) d_fl[it].ptr;
+        s.nstag = T.nstag;
+        s.nlvl0 = T.nlvl0;
+        s.stag_hist_nelts = T.stag_hist_nelts;
+        s.stag_begin = (const int *) d_stag_begin[it].ptr;
+        s.M = T.M;
+        s.pf_rank = int(t.frequency_subbands.pf_rank);
+        s.state = (float *) s2_state[it].ptr;
+        s.state_bstride = s2_state_bstride[it];
+        s.B = int(B);
+        s.amb = t.amb_rank;
+        s.dd = t.dd_rank;
+        s.Tp = t.nt_ds;
+        s.beam0 = beam0;
+        s.amb1_dd1 = t.amb_rank;
+        s.chunk = ichunk;
+        hipEvent_t ev;
+        const std::string nm2 = "stage2_t" + std::to_string(it);
+        timed_begin(nm2, st, ev);
+        launch_stage2(dtype, s, st);
+        timed_end(nm2, st, ev);
+
+        PfArgs f;
+        f.sb = sb[it].ptr;
+        f.sb_bstride = s.sb_bstride;
+        f.rowlen = T.rowlen;
+        f.tpad = T.tpad;
+        f.flmax = T.flmax;
+        f.m_to_n = (const int *) d_m_to_n[it].ptr;
+        const long wslot_n = B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
+        f.wt = (const float *) weights[it].ptr + wslot * wslot_n;
+        f.wt_bstride = t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
+        const long oslot_n = B * t.ndm_out * t.nt_out;
+        f.out_max = (char *) out_max[it].ptr + oslot * oslot_n * elt_bytes;
+        f.out_argmax = (unsigned int *) out_argmax[it].ptr + oslot * oslot_n;
+        f.out_bstride = t.ndm_out * t.nt_out;
+        f.B = int(B);
+        f.ndm_out = int(t.ndm_out);
+        f.M = T.M;
+        f.N = T.N;
+        f.P = int(t.nprofiles);
+        f.nlev = T.nlev;
+        f.Tp = t.nt_ds;
+        f.Dout = int(t.pf.time_downsampling);
+        f.Dcore = int(t.Dcore);
+        f.ndm_wt = int(t.ndm_wt);
+        f.nt_wt = int(t.nt_wt);
+        f.nt_out = int(t.nt_out);
+        f.beam0 = beam0;
+        const std::string nmp = "pf_t" + std::to_string(it);
+        timed_begin(nmp, st, ev);
+        launch_pf(dtype, f, st);
+        timed_end(nmp, st, ev);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+
+void GpuDedisperser::check_stopped_locked()
+{
+    if (stopped)
+        throw pdd_error("GpuDedisperser was stopped: " + stop_reason);
+}
+
+
+void GpuDedisperser::stop(const std::string &reason)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    if (!stopped) {
+        stopped = true;
+        stop_reason = reason.empty() ? "stop() called" : reason;
+    }
+    cv.notify_all();
+}
+
+
+bool GpuDedisperser::is_stopped()
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    return stopped;
+}
+
+
+// Reference: src_lib/GpuDedisperser.cpp:742-823 (cursor checks; slot = seq_id % nstreams;
+// the caller's stream waits until the slot's previous contents have been consumed).
+void *GpuDedisperser::acquire_input(long seq_id, hipStream_t stream)
+{
+    std::unique_lock<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(is_allocated, "GpuDedisperser::acquire_input(): allocate() has not been called");
+    if (seq_id != curr_input_acquire || curr_input_acquire != curr_input_release) {
+        std::ostringstream ss;
+        ss << "GpuDedisperser::acquire_input(): expected seq_id=" << curr_input_release << ", got " << seq_id
+           << " (acquire/release must alternate, in order)";
+        stopped = true;
+        stop_reason = ss.str();
+        cv.notify_all();
+        throw pdd_error(ss.str());
+    }
+    const long slot = seq_id % nstreams;
+    HIP_CHECK(hipStreamWaitEvent(stream, ev_input_free[slot], 0));
+    curr_input_acquire++;
+    return (char *) input_slots.ptr + size_t(slot) * beams_per_batch * nfreq * nt_in * elt_bytes;
+}
+
+
+// Reference: src_lib/GpuDedisperser.cpp:826-855, 990-1105. Back-pressure: the host blocks
+// until every consumer has released output seq_id - nbatches_out.
+void GpuDedisperser::release_input_and_launch(long seq_id, hipStream_t stream)
+{
+    std::unique_lock<std::mutex> lk(mutex);
+    check_stopped_locked();
+    if (seq_id != curr_input_release || curr_input_acquire != curr_input_release + 1) {
+        std::ostringstream ss;
+        ss << "GpuDedisperser::release_input_and_launch(): expected seq_id=" << curr_input_release << ", got " << seq_id;
+        stopped = true;
+        stop_reason = ss.str();
+        cv.notify_all();
+        throw pdd_error(ss.str());
+    }
+    const long oslot = seq_id % nbatches_out;
+    cv.wait(lk, [&] {
+        if (stopped)
+            return true;
+        for (long c = 0; c < num_consumers; c++)
+            if (curr_output_release[c] < seq_id - nbatches_out + 1)
+                return false;
+        return true;
+    });
+    check_stopped_locked();
+
+    const long slot = seq_id % nstreams;
+    const long ichunk = seq_id / nbatches, ibatch = seq_id % nbatches;
+    HIP_CHECK(hipEventRecord(ev_input_ready[slot], stream));
+    HIP_CHECK(hipStreamWaitEvent(compute_stream, ev_input_ready[slot], 0));
+    for (long c = 0; c < num_consumers; c++)
+        HIP_CHECK(hipStreamWaitEvent(compute_stream, ev_output_released[oslot * num_consumers + c], 0));
+
+    const void *in = (char *) input_slots.ptr + size_t(slot) * beams_per_batch * nfreq * nt_in * elt_bytes;
+    launch_chain(in, ichunk, ibatch, oslot, seq_id % nbatches_wt, compute_stream);
+
+    HIP_CHECK(hipEventRecord(ev_input_free[slot], compute_stream));
+    HIP_CHECK(hipEventRecord(ev_output_ready[oslot], compute_stream));
+    curr_input_release++;
+    curr_launched = seq_id + 1;
+    cv.notify_all();
+}
+
+
+// Reference: src_lib/GpuDedisperser.cpp:858-935.
+void GpuDedisperser::acquire_output(long consumer, long seq_id, hipStream_t stream, bool sync,
+                                    std::vector<void *> &omax, std::vector<unsigned int *> &oarg)
+{
+    std::unique_lock<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(consumer >= 0 && consumer < num_consumers, "acquire_output(): consumer out of range");
+    long &acq = curr_output_acquire[consumer];
+    long &rel = curr_output_release[consumer];
+    bool ok = (seq_id == acq) && (params.synchronous ? (acq == rel) : true);
+    if (!ok) {
+        std::ostringstream ss;
+        ss << "GpuDedisperser::acquire_output(): consumer " << consumer << " expected seq_id=" << acq << ", got " << seq_id;
+        stopped = true;
+        stop_reason = ss.str();
+        cv.notify_all();
+        throw pdd_error(ss.str());
+    }
+    cv.wait(lk, [&] { return stopped || curr_launched > seq_id; });
+    check_stopped_locked();
+
+    const long oslot = seq_id % nbatches_out;
+    HIP_CHECK(hipStreamWaitEvent(stream, ev_output_ready[oslot], 0));
+    if (sync)
+        HIP_CHECK(hipEventSynchronize(ev_output_ready[oslot]));
+    acq++;
+
+    omax.resize(ntrees);
+    oarg.resize(ntrees);
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        const long n = beams_per_batch * t.ndm_out * t.nt_out;
+        omax[it] = (char *) out_max[it].ptr + oslot * n * elt_bytes;
+        oarg[it] = (unsigned int *) out_argmax[it].ptr + oslot * n;
+    }
+}
+
+
+void GpuDedisperser::release_output(long consumer, long seq_id, hipStream_t stream)
+{
+    std::unique_lock<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(consumer >= 0 && consumer < num_consumers, "release_output(): consumer out of range");
+    long &acq = curr_output_acquire[consumer];
+    long &rel = curr_output_release[consumer];
+    if (seq_id != rel || rel >= acq) {
+        std::ostringstream ss;
+        ss << "GpuDedisperser::release_output(): consumer " << consumer << " expected seq_id=" << rel << ", got " << seq_id;
+        stopped = true;
+        stop_reason = ss.str();
+        cv.notify_all();
+        throw pdd_error(ss.str());
+    }
+    const long oslot = seq_id % nbatches_out;
+    HIP_CHECK(hipEventRecord(ev_output_released[oslot * num_consumers + consumer], stream));
+    rel++;
+    cv.notify_all();
+}
+
+
+// Reference: GpuDedisperser::fill_all_weights (src_lib/GpuDedisperser.cpp:1257-1279).
+// This build's peak finder takes the per-group maximum before weighting, which equals the
+// maximum of the weighted candidates only for non-negative weights; negative or non-finite
+// weights are rejected.
+void GpuDedisperser::fill_all_weights(long itree, const float *host, long nelts)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(is_allocated, "fill_all_weights(): allocate() has not been called");
+    PDD_CHECK(itree >= 0 && itree < ntrees, "fill_all_weights(): itree out of range");
+    const DedispersionTree &t = plan->trees[itree];
+    long expect = nbatches_wt * beams_per_batch * t.ndm_wt * t.nt_wt * t.nprofiles * tables[itree].N;
+    PDD_CHECK(nelts == expect, "fill_all_weights(): expected " << expect << " weights (nbatches_wt, B, ndm_wt, nt_wt, P, N), got " << nelts);
+    for (long i = 0; i < nelts; i++)
+        PDD_CHECK(std::isfinite(host[i]) && host[i] >= 0.0f, "fill_all_weights(): weights must be finite and >= 0 (index " << i << ")");
+    HIP_CHECK(hipMemcpy(weights[itree].ptr, host, nelts * sizeof(float), hipMemcpyHostToDevice));
+}
+
+
+hipEvent_t GpuDedisperser::timing_event()
+{
+    if (!timing_event_pool.empty()) {
+        hipEvent_t e = timing_event_pool.back();
+        timing_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void GpuDedisperser::timed_begin(const std::string &, hipStream_t st, hipEvent_t &ev)
+{
+    ev = nullptr;
+    if (!timing)
+        return;
+    ev = timing_event();
+    HIP_CHECK(hipEventRecord(ev, st));
+}
+
+void GpuDedisperser::timed_end(const std::string &name, hipStream_t st, hipEvent_t ev)
+{
+    if (!timing || !ev)
+        return;
+    hipEvent_t e2 = timing_event();
+    HIP_CHECK(hipEventRecord(e2, st));
+    timed_pending.push_back({name, ev, e2});
+    if (timed_pending.size() > 4096)
+        collect_timing_locked();
+}
+
+void GpuDedisperser::collect_timing_locked()
+{
+    for (TimedLaunch &t : timed_pending) {
+        HIP_CHECK(hipEventSynchronize(t.stop));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, t.start, t.stop));
+        bool found = false;
+        for (auto &kv : timing_totals)
+            if (kv.first == t.name) {
+                kv.second.first++;
+                kv.second.second += ms;
+                found = true;
+            }
+        if (!found)
+            timing_totals.push_back({t.name, {1, double(ms)}});
+        timing_event_pool.push_back(t.start);
+        timing_event_pool.push_back(t.stop);
+    }
+    timed_pending.clear();
+}
+
+void GpuDedisperser::set_kernel_timing(bool enable)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    collect_timing_locked();
+    timing = enable;
+    timing_totals.clear();
+}
+
+std::string GpuDedisperser::kernel_timing_report()
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    collect_timing_locked();
+    std::ostringstream os;
+    for (auto &kv : timing_totals)
+        os << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
+    return os.str();
+}
+
+
+size_t GpuDedisperser::debug_copy(const std::string &name, void *host, size_t maxbytes)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    PDD_CHECK(is_allocated, "debug_copy(): not allocated");
+    HIP_CHECK(hipStreamSynchronize(compute_stream));
+    const DeviceArray *a = nullptr;
+    if (name == "grid")
+        a = &grid_buf;
+    else if (name.rfind("lds", 0) == 0)
+        a = &lds_out.at(std::stol(name.substr(3)));
+    else if (name.rfind("sb", 0) == 0)
+        a = &sb.at(std::stol(name.substr(2)));
+    else if (name.rfind("ring", 0) == 0)
+        a = &rings.at(std::stol(name.substr(4)));
+    PDD_CHECK(a, "debug_copy(): unknown buffer '" << name << "'");
+    if (host) {
+        PDD_CHECK(maxbytes >= a->nbytes, "debug_copy(): host buffer too small");
+        HIP_CHECK(hipMemcpy(host, a->ptr, a->nbytes, hipMemcpyDeviceToHost));
+    }
+    return a->nbytes;
+}
+
+}  // namespace pirate_amd

@@ -1,0 +1,146 @@
+// GpuDedisperser: the MI355X dedispersion chain behind the reference's acquire/release API.
+// Reference interface: include/pirate/Dedisperser.hpp:45-392 (GpuDedisperser),
+// src_lib/GpuDedisperser.cpp:742-984 (acquire_input / release_input_and_launch_dd_kernels /
+// acquire_output / release_output cursors, back-pressure and stop semantics).
+//
+// One compute stream per dedisperser runs, per (chunk, beam batch):
+//   tree gridding -> lagged downsampling -> stage-1 trees -> lag ring ->
+//   stage-2 trees + subband streams -> peak finding -> output ring slot.
+// The caller's streams synchronize with it through events only.
+
+#ifndef PIRATE_AMD_DEDISPERSER_HPP
+#define PIRATE_AMD_DEDISPERSER_HPP
+
+#include "../host/plan.hpp"
+#include "kernels.hpp"
+
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace pirate_amd {
+
+struct GpuDedisperserParams {
+    long nbatches_out = 0;     // output ring slots (0 = num_active_batches)
+    long nbatches_wt = 0;      // weight ring slots (0 = nbatches_out)
+    long num_consumers = 1;
+    int device = 0;
+    bool synchronous = false;
+    long initial_chunk = 0;
+};
+
+struct DeviceArray {
+    void *ptr = nullptr;
+    size_t nbytes = 0;
+};
+
+class GpuDedisperser {
+public:
+    GpuDedisperser(std::shared_ptr<DedispersionPlan> plan, const GpuDedisperserParams &params);
+    ~GpuDedisperser();
+    GpuDedisperser(const GpuDedisperser &) = delete;
+    GpuDedisperser &operator=(const GpuDedisperser &) = delete;
+
+    std::shared_ptr<DedispersionPlan> plan;
+    GpuDedisperserParams params;
+
+    Dtype dtype;
+    long nfreq, nt_in, total_beams, beams_per_batch, nbatches, nstreams, ntrees;
+    long nbatches_out, nbatches_wt, num_consumers;
+    size_t elt_bytes;
+
+    void allocate();
+    bool is_allocated = false;
+    size_t gpu_bytes_allocated = 0;
+
+    // Returns the device address of the (B, nfreq, nt_in) input slot for seq_id.
+    void *acquire_input(long seq_id, hipStream_t stream);
+    void release_input_and_launch(long seq_id, hipStream_t stream);
+
+    // out_max[itree], out_argmax[itree]: device addresses of (B, ndm_out, nt_out) arrays.
+    void acquire_output(long consumer, long seq_id, hipStream_t stream, bool sync,
+                        std::vector<void *> &out_max, std::vector<unsigned int *> &out_argmax);
+    void release_output(long consumer, long seq_id, hipStream_t stream);
+
+    // host: (nbatches_wt, B, ndm_wt, nt_wt, P, N) float32, all entries finite and >= 0.
+    void fill_all_weights(long itree, const float *host, long nelts);
+
+    void stop(const std::string &reason);
+    bool is_stopped();
+
+    // Launches the chain for (ichunk, ibatch) on 'stream' without the acquire/release
+    // protocol; the input is read from 'input' (device, (B, nfreq, nt_in), storage dtype)
+    // and the outputs go to output slot 'oslot'. Used by the timing harness.
+    void launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t stream);
+
+    // Copies an internal buffer to the host (tests): name in {"grid", "lds<p>", "sb<t>"}.
+    size_t debug_copy(const std::string &name, void *host, size_t maxbytes);
+
+    // Per-kernel timing with HIP events recorded on the compute stream around every kernel
+    // launch (kernel names: gridding, lds, stage1_p<p>, stage2_t<i>, pf_t<i>). The report
+    // lists "name launches total_ms" lines, accumulated since timing was enabled.
+    void set_kernel_timing(bool enable);
+    std::string kernel_timing_report();
+
+    hipStream_t compute_stream = nullptr;
+
+    // Per-tree stage-2 tables (host copies, exposed for tests).
+    struct TreeTables {
+        std::vector<int> main_emit, stag, lvl0, fl, m_to_n, stag_begin;
+        int nstag = 0, nlvl0 = 0, stag_hist_nelts = 0, flmax = 0, tpad = 0, nlev = 0, rowlen = 0, M = 0, N = 0;
+    };
+    std::vector<TreeTables> tables;
+
+private:
+    std::mutex mutex;
+    std::condition_variable cv;
+    bool stopped = false;
+    std::string stop_reason;
+
+    long curr_input_acquire = 0, curr_input_release = 0;
+    std::vector<long> curr_output_acquire, curr_output_release;
+
+    DeviceArray input_slots;          // (nstreams, B, nfreq, nt_in)
+    DeviceArray grid_buf;             // (B, 2^R, nt_in)
+    std::vector<DeviceArray> lds_out; // [p] (B, 2^(R-1), nt_in >> p), p >= 1
+    std::vector<DeviceArray> rings;   // [p]
+    std::vector<DeviceArray> ring_row_off, ring_row_cap;   // [p] device (long)
+    std::vector<long> ring_bstride;
+    std::vector<DeviceArray> stage2_lags;    // [itree] device int
+    std::vector<DeviceArray> sb;      // [itree] (B, ndm_out, M, rowlen)
+    std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin;
+    DeviceArray lds_state;  long lds_state_bstride = 0;
+    std::vector<DeviceArray> s1_state; std::vector<long> s1_state_bstride;
+    std::vector<DeviceArray> s2_state; std::vector<long> s2_state_bstride;
+    std::vector<DeviceArray> weights; // [itree] (nbatches_wt, B, ndm_wt, nt_wt, P, N) float
+    std::vector<DeviceArray> out_max, out_argmax;   // [itree] (nbatches_out, B, ndm_out, nt_out)
+    DeviceArray grid_rows;            // int (3*nchan) f0, cnt, woff
+    DeviceArray grid_w;               // float
+
+    std::vector<hipEvent_t> ev_input_free;        // [slot]
+    std::vector<hipEvent_t> ev_input_ready;       // [slot]
+    std::vector<hipEvent_t> ev_output_ready;      // [oslot]
+    std::vector<hipEvent_t> ev_output_released;   // [oslot * num_consumers + c]
+    long curr_launched = 0;
+
+    bool timing = false;
+    struct TimedLaunch { std::string name; hipEvent_t start, stop; };
+    std::vector<TimedLaunch> timed_pending;
+    std::vector<hipEvent_t> timing_event_pool;
+    std::vector<std::pair<std::string, std::pair<long, double>>> timing_totals;   // name -> (launches, ms)
+    hipEvent_t timing_event();
+    void timed_begin(const std::string &name, hipStream_t st, hipEvent_t &ev);
+    void timed_end(const std::string &name, hipStream_t st, hipEvent_t ev);
+    void collect_timing_locked();
+
+    DeviceArray dalloc(size_t nbytes);
+    void check_stopped_locked();
+    void build_tables();
+    void validate_gpu_limits();
+};
+
+}  // namespace pirate_amd
+
+#endif

@@ -1,0 +1,41 @@
+// Device-side helpers shared by the dedispersion kernels (gfx950, wave64).
+//
+// Storage types: float, __half, __hip_bfloat16. All arithmetic is float32; the
+// 16-bit types are storage only (converted with round-to-nearest-even on store).
+// Every kernel is compiled with -ffp-contract=off: each multiply and add rounds
+// separately, which makes the float32 path bit-identical to the CPU oracle.
+
+#ifndef PIRATE_AMD_DEVICE_COMMON_HPP
+#define PIRATE_AMD_DEVICE_COMMON_HPP
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace pirate_amd {
+
+static constexpr float kRsqrt2 = 0.7071067811865476f;
+
+__device__ __forceinline__ float ld_st(const float *p) { return *p; }
+__device__ __forceinline__ float ld_st(const __half *p) { return __half2float(*p); }
+__device__ __forceinline__ float ld_st(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+
+__device__ __forceinline__ void st_st(float *p, float x) { *p = x; }
+__device__ __forceinline__ void st_st(__half *p, float x) { *p = __float2half(x); }
+__device__ __forceinline__ void st_st(__hip_bfloat16 *p, float x) { *p = __float2bfloat16(x); }
+
+__device__ __forceinline__ int bitrev_dev(int i, int nbits)
+{
+    return (nbits > 0) ? int(__brev(unsigned(i)) >> (32 - nbits)) : 0;
+}
+
+// Position of time tau in a circular ring row of capacity C (tau may be negative).
+__device__ __forceinline__ long ring_pos(long tau, long C)
+{
+    long r = tau % C;
+    return (r < 0) ? (r + C) : r;
+}
+
+}  // namespace pirate_amd
+
+#endif

@@ -1,0 +1,366 @@
+// Tree dedispersion on gfx950 (stage 1 and stage 2 of the two-stage tree).
+//
+// Reference semantics: ReferenceTree (src_lib/ReferenceTree.cpp:120-345) applied as in
+// ReferenceDedisperser sophistication 1 (src_lib/ReferenceDedisperser.cpp:388-450):
+//   pass r pairs rows e = fout*2^(r+1) + dmb and o = e + 2^r (in-place, bit-reversed DM):
+//     e <- c*(x_e[t - L] + x_o[t]),  o <- c*(x_e[t - L - 1] + x_o[t]),  L = bitrev(dmb, r),
+//   with x_e at negative chunk time taken from the per-pass history of the previous chunk.
+//
+// Layout: one workgroup per tree instance (beam, ambient index) sweeps the chunk in time
+// tiles of TT = 64 samples (one wave64 row segment). The tile [D][TT] and the per-pass
+// histories (D/2 pairs x 2^r samples at pass r) live in LDS; passes alternate a read
+// phase (all lags resolved from tile or history into registers) and a write phase.
+// Histories persist across chunks in an HBM state buffer (loaded/saved once per chunk).
+//
+// Stage 1 writes its output rows to the lag ring (per-row circular buffers whose
+// capacity covers the row's stage-2 lag). Stage 2 reads ring rows at their full lag
+// (rb_lag, no segment/residual split), runs the stage-2 passes, and emits the subband
+// streams (ReferenceTree subband outputs, incl. the final lags) to the peak-finder input.
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace pirate_amd {
+
+static constexpr int kTT = 64;
+static constexpr int kTreeThreads = 256;
+
+template<int DD>
+struct TreeDims {
+    static constexpr int D = 1 << DD;
+    static constexpr int NP = D / 2;
+    static constexpr int HIST = NP * (D - 1);            // sum_r NP * 2^r
+    static constexpr int WORK = NP * kTT;                 // pair-samples per pass
+    static constexpr int ITEMS = (WORK + kTreeThreads - 1) / kTreeThreads;
+};
+
+__device__ __forceinline__ int hist_off(int np, int r) { return np * ((1 << r) - 1); }
+
+
+// One in-place pass. If emit != nullptr, the outputs of main pairs with emit[p] >= 0
+// are also sent to streams emit[p] (out0) and emit[p]+1 (out1) through 'sink'.
+template<int DD, typename Sink>
+__device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const int *emit, Sink &sink)
+{
+    using TD = TreeDims<DD>;
+    const int tid = threadIdx.x;
+    const int half = 1 << r;
+    float *h = hist + hist_off(TD::NP, r);
+
+    float o0[TD::ITEMS], o1[TD::ITEMS], sv[TD::ITEMS];
+#pragma unroll
+    for (int i = 0; i < TD::ITEMS; i++) {
+        const int idx = i * kTreeThreads + tid;
+        if (idx < TD::WORK) {
+            const int p = idx / kTT, t = idx % kTT;
+            const int dmb = p & (half - 1);
+            const int e = ((p >> r) << (r + 1)) + dmb;
+            const int o = e + half;
+            const int L = bitrev_dev(dmb, r);
+            const float xo = tile[o * kTT + t];
+            const float xa = (t >= L) ? tile[e * kTT + t - L] : h[p * half + half + t - L];
+            const float xb = (t >= L + 1) ? tile[e * kTT + t - L - 1] : h[p * half + half + t - L - 1];
+            o0[i] = kRsqrt2 * (xa + xo);
+            o1[i] = kRsqrt2 * (xb + xo);
+            sv[i] = (t >= kTT - half) ? tile[e * kTT + t] : 0.0f;
+        }
+    }
+    sink.read_phase_extra(r);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TD::ITEMS; i++) {
+        const int idx = i * kTreeThreads + tid;
+        if (idx < TD::WORK) {
+            const int p = idx / kTT, t = idx % kTT;
+            const int dmb = p & (half - 1);
+            const int e = ((p >> r) << (r + 1)) + dmb;
+            const int o = e + half;
+            tile[e * kTT + t] = o0[i];
+            tile[o * kTT + t] = o1[i];
+            if (t >= kTT - half)
+                h[p * half + (t - (kTT - half))] = sv[i];
+            if (emit) {
+                const int s0 = emit[r * TD::NP + p];
+                if (s0 >= 0) {
+                    sink.emit(s0, t, o0[i]);
+                    sink.emit(s0 + 1, t, o1[i]);
+                }
+            }
+        }
+    }
+    sink.write_phase_extra(r);
+    __syncthreads();
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Stage 1
+
+
+long stage1_state_nelts_per_tree(int dd) { return (1L << (dd - 1)) * ((1L << dd) - 1); }
+
+struct NoSink {
+    __device__ void read_phase_extra(int) { }
+    __device__ void write_phase_extra(int) { }
+    __device__ void emit(int, int, float) { }
+};
+
+template<typename Tst, int DD>
+__global__ void __launch_bounds__(kTreeThreads)
+k_stage1(Stage1Args a)
+{
+    using TD = TreeDims<DD>;
+    extern __shared__ float smem[];
+    float *tile = smem;
+    float *hist = smem + TD::D * kTT;
+
+    const int ia = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x;
+    float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(ia) * TD::HIST;
+
+    for (int i = tid; i < TD::HIST; i += kTreeThreads)
+        hist[i] = st[i];
+
+    const Tst *in = reinterpret_cast<const Tst *>(a.in) + long(b) * a.in_bstride + long(ia) * TD::D * a.Tp;
+    Tst *ring = reinterpret_cast<Tst *>(a.ring.base) + long(a.beam0 + b) * a.ring.bstride;
+    NoSink sink;
+
+    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
+        for (int idx = tid; idx < TD::D * kTT; idx += kTreeThreads) {
+            const int row = idx / kTT, t = idx % kTT;
+            tile[idx] = ld_st(in + long(row) * a.Tp + t0 + t);
+        }
+        __syncthreads();
+
+#pragma unroll 1
+        for (int r = 0; r < DD; r++)
+            tree_pass<DD>(r, tile, hist, nullptr, sink);
+
+        // Output row i (bit-reversed stage-1 DM) -> ring row (ia, i). One wave per row
+        // segment; ring capacities are multiples of 64, so a tile never wraps.
+        const long tau0 = a.chunk * a.Tp + t0;
+        const int lane = tid & 63;
+        for (int row = __builtin_amdgcn_readfirstlane(tid >> 6); row < TD::D; row += kTreeThreads / 64) {
+            const long rr = long(ia) * TD::D + row;
+            const long cap = a.ring.row_cap[rr];
+            const long pos = tau0 % cap;
+            st_st(ring + a.ring.row_off[rr] + pos + lane, tile[row * kTT + lane]);
+        }
+        __syncthreads();
+    }
+
+    for (int i = tid; i < TD::HIST; i += kTreeThreads)
+        st[i] = hist[i];
+}
+
+template<typename Tst>
+static void launch_stage1_t(const Stage1Args &a, hipStream_t st)
+{
+    dim3 grid(1 << a.amb, a.B);
+    dim3 block(kTreeThreads);
+#define PDD_S1(DDV)                                                                           \
+    case DDV: {                                                                               \
+        size_t shm = sizeof(float) * (TreeDims<DDV>::D * kTT + TreeDims<DDV>::HIST);          \
+        hipFuncSetAttribute((const void *) k_stage1<Tst, DDV>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)); \
+        hipLaunchKernelGGL((k_stage1<Tst, DDV>), grid, block, shm, st, a);                    \
+        break;                                                                                \
+    }
+    switch (a.dd) {
+        PDD_S1(1) PDD_S1(2) PDD_S1(3) PDD_S1(4) PDD_S1(5) PDD_S1(6) PDD_S1(7)
+        default: break;
+    }
+#undef PDD_S1
+}
+
+void launch_stage1(Dtype dt, const Stage1Args &a, hipStream_t st)
+{
+    switch (dt) {
+        case Dtype::float32: launch_stage1_t<float>(a, st); break;
+        case Dtype::float16: launch_stage1_t<__half>(a, st); break;
+        case Dtype::bfloat16: launch_stage1_t<__hip_bfloat16>(a, st); break;
+    }
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Stage 2
+
+
+long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts)
+{
+    return (1L << (dd - 1)) * ((1L << dd) - 1) + stag_hist_nelts;
+}
+
+int stage2_lds_bytes(int dd, int stag_hist_nelts)
+{
+    long d = 1L << dd;
+    return int(sizeof(float) * (d * kTT + (d / 2) * (d - 1) + stag_hist_nelts));
+}
+
+// Emits subband stream samples (and runs the staggered butterflies and level-0 copies)
+// into the peak-finder input rows. Holds copies of the fields it needs (not a reference
+// to the kernel argument, which would force the argument block into scratch memory).
+template<typename Tst, int DD>
+struct Stage2Sink {
+    Tst *sbb;            // sb + beam offset
+    const int *stag, *fl;
+    int M, rowlen, tpad;
+    int dpf_base, t0;
+    const int *stag_begin;
+    float *tile, *shist;
+
+    // Staggered outputs computed in the read phase, written in the write phase.
+    // At most 16 staggered pairs per pass (host-checked): 16*64/256 = 4 items per thread.
+    static constexpr int SMAX = 4;
+    float so0[SMAX], so1[SMAX], ssv[SMAX];
+
+    __device__ __forceinline__ void emit(int s, int t, float v)
+    {
+        const int pf_dm = s / M, m = s - pf_dm * M;
+        const long row = long(dpf_base + pf_dm) * M + m;
+        st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
+    }
+
+    __device__ __forceinline__ void read_phase_extra(int r)
+    {
+        const int half = 1 << r;
+        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+#pragma unroll
+        for (int q = 0; q < SMAX; q++) {
+            const int idx = q * kTreeThreads + threadIdx.x;
+            if (idx < n) {
+                const int k = k0 + idx / kTT, t = idx % kTT;
+                const int *e = stag + 6 * k;
+                const int row0 = e[1], row1 = e[2], L = e[3];
+                const float *h = shist + e[5];
+                const float xo = tile[row1 * kTT + t];
+                const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
+                const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
+                so0[q] = kRsqrt2 * (xa + xo);
+                so1[q] = kRsqrt2 * (xb + xo);
+                ssv[q] = tile[row0 * kTT + t];
+            }
+        }
+    }
+
+    __device__ __forceinline__ void write_phase_extra(int r)
+    {
+        const int half = 1 << r;
+        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+#pragma unroll
+        for (int q = 0; q < SMAX; q++) {
+            const int idx = q * kTreeThreads + threadIdx.x;
+            if (idx < n) {
+                const int k = k0 + idx / kTT, t = idx % kTT;
+                const int *e = stag + 6 * k;
+                emit(e[4], t, so0[q]);
+                emit(e[4] + 1, t, so1[q]);
+                if (t >= kTT - half)
+                    shist[e[5] + (t - (kTT - half))] = ssv[q];
+            }
+        }
+    }
+};
+
+template<typename Tst, int DD>
+__global__ void __launch_bounds__(kTreeThreads)
+k_stage2(Stage2Args a)
+{
+    using TD = TreeDims<DD>;
+    extern __shared__ float smem[];
+    float *tile = smem;
+    float *hist = smem + TD::D * kTT;
+    float *shist = hist + TD::HIST;
+
+    const int j = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x;
+    const long nst = TD::HIST + a.stag_hist_nelts;
+    float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(j) * nst;
+
+    for (int i = tid; i < nst; i += kTreeThreads)
+        hist[i] = st[i];     // hist and shist are contiguous
+
+    const int pf_ndm = 1 << (DD - a.pf_rank);
+    const int dpf_base = bitrev_dev(j, a.amb) * pf_ndm;
+    const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + long(a.beam0 + b) * a.ring.bstride;
+    Tst *sbb = reinterpret_cast<Tst *>(a.sb) + long(a.beam0 + b) * a.sb_bstride;
+    Stage2Sink<Tst, DD> sink;
+    sink.sbb = sbb;
+    sink.stag = a.stag;
+    sink.fl = a.fl;
+    sink.M = a.M;
+    sink.rowlen = a.rowlen;
+    sink.tpad = a.tpad;
+    sink.dpf_base = dpf_base;
+    sink.t0 = 0;
+    sink.tile = tile;
+    sink.shist = shist;
+    sink.stag_begin = a.stag_begin;
+    const int r_lvl0 = DD - a.pf_rank;
+
+    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
+        sink.t0 = t0;
+        const long tau0 = a.chunk * a.Tp + t0;
+        const int lane = tid & 63;
+        for (int fc = __builtin_amdgcn_readfirstlane(tid >> 6); fc < TD::D; fc += kTreeThreads / 64) {
+            const long rr = (long(fc) << a.amb) + j;
+            const long cap = a.ring.row_cap[rr];
+            long pos = ring_pos(tau0 - a.row_lag[rr], cap) + lane;
+            if (pos >= cap)
+                pos -= cap;
+            tile[fc * kTT + lane] = ld_st(ring + a.ring.row_off[rr] + pos);
+        }
+        __syncthreads();
+
+#pragma unroll 1
+        for (int r = 0; r < DD; r++) {
+            if (r == r_lvl0) {
+                for (int idx = tid; idx < a.nlvl0 * kTT; idx += kTreeThreads) {
+                    const int k = idx / kTT, t = idx % kTT;
+                    sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
+                }
+            }
+            tree_pass<DD>(r, tile, hist, a.main_emit, sink);
+        }
+        if (r_lvl0 == DD) {
+            for (int idx = tid; idx < a.nlvl0 * kTT; idx += kTreeThreads) {
+                const int k = idx / kTT, t = idx % kTT;
+                sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
+            }
+            __syncthreads();
+        }
+    }
+
+    for (int i = tid; i < nst; i += kTreeThreads)
+        st[i] = hist[i];
+}
+
+template<typename Tst>
+static void launch_stage2_t(const Stage2Args &a, hipStream_t st)
+{
+    dim3 grid(1 << a.amb, a.B);
+    dim3 block(kTreeThreads);
+    size_t shm = stage2_lds_bytes(a.dd, a.stag_hist_nelts);
+#define PDD_S2(DDV)                                                                           \
+    case DDV: {                                                                               \
+        hipFuncSetAttribute((const void *) k_stage2<Tst, DDV>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)); \
+        hipLaunchKernelGGL((k_stage2<Tst, DDV>), grid, block, shm, st, a);                    \
+        break;                                                                                \
+    }
+    switch (a.dd) {
+        PDD_S2(1) PDD_S2(2) PDD_S2(3) PDD_S2(4) PDD_S2(5) PDD_S2(6) PDD_S2(7)
+        default: break;
+    }
+#undef PDD_S2
+}
+
+void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st)
+{
+    switch (dt) {
+        case Dtype::float32: launch_stage2_t<float>(a, st); break;
+        case Dtype::float16: launch_stage2_t<__half>(a, st); break;
+        case Dtype::bfloat16: launch_stage2_t<__hip_bfloat16>(a, st); break;
+    }
+}
+
+}  // namespace pirate_amd

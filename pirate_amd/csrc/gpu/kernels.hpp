@@ -1,0 +1,109 @@
+// Host-visible argument structs and launchers for the gfx950 dedispersion kernels.
+// All pointers are device pointers; strides and offsets are in elements.
+
+#ifndef PIRATE_AMD_KERNELS_HPP
+#define PIRATE_AMD_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+#include "../host/config.hpp"
+
+namespace pirate_amd {
+
+// ---- tree gridding: out[b,n,t] = sum_f w[n,f] in[b,f,t] (CSR weights, ascending f) ----
+struct GriddingArgs {
+    const void *in;
+    void *out;
+    long in_bstride, out_bstride;
+    int B, nchan, nfreq, ntime;
+    const int *row_f0;      // (nchan) first input channel of tree channel n
+    const int *row_cnt;     // (nchan) number of input channels
+    const int *row_woff;    // (nchan) offset into w
+    const float *w;         // overlap weights
+};
+void launch_gridding(Dtype dt, const GriddingArgs &a, hipStream_t s);
+
+
+// ---- lagged downsampling, primary trees p = 1..npri-1 ----
+static constexpr int kMaxPrimaryTrees = 8;
+
+struct LdsArgs {
+    const void *in;                      // (B, 2^R, T) level-0 tree-gridded
+    long in_bstride;
+    void *out[kMaxPrimaryTrees];         // out[p], p >= 1: (B, 2^(R-1), T >> p)
+    long out_bstride[kMaxPrimaryTrees];
+    float *state;                        // per global beam, see lds_state_nelts()
+    long state_bstride;
+    int B, R, npri, ntime, s;            // s = (stage-1 dd_rank of p>=1) + 1
+    int beam0;                           // global index of the batch's first beam
+};
+long lds_state_nelts(int R, int npri, int s);   // per beam
+void launch_lds(Dtype dt, const LdsArgs &a, hipStream_t st);
+
+
+// ---- stage-1 tree: (B, 2^amb, 2^dd, Tp) -> lag ring rows (a, dm_brev) ----
+struct RingView {
+    void *base;               // ring storage of one primary tree
+    const long *row_off;      // (nrows) element offset of each row within a beam
+    const long *row_cap;      // (nrows) capacity (elements) of each row
+    long bstride;             // elements per beam
+};
+
+struct Stage1Args {
+    const void *in;
+    long in_bstride;
+    RingView ring;
+    float *state;             // per (global beam, a): tree pass history
+    long state_bstride;       // per beam
+    int B, amb, dd, Tp, beam0;
+    long chunk;               // chunk index (ring time = chunk*Tp + t)
+};
+long stage1_state_nelts_per_tree(int dd);
+void launch_stage1(Dtype dt, const Stage1Args &a, hipStream_t st);
+
+
+// ---- stage-2 tree + subband extraction -> peak-finder input rows ----
+// Per-tree device tables built on the host (see dedisperser.cpp):
+//   main_emit[r*(D/2) + p]: first stream (pf_dm*M + m) emitted by main pair p of pass r, or -1
+//   stag[k*6 + 0..5]: (pass r, row0, row1, lag, stream s0, hist offset) staggered butterflies
+//   lvl0[k*2 + 0..1]: (row, stream) copies of the pass (dd - pf_rank) input
+//   fl[s]: final lag of stream s (applied as a write offset)
+struct Stage2Args {
+    RingView ring;            // ring of the tree's primary tree
+    const int *row_lag;       // (2^dd rows fc, 2^amb j) -> lag, indexed fc*2^amb + j
+    void *sb;                 // (B, ndm_out, M, rowlen) peak-finder input rows
+    long sb_bstride;
+    int rowlen, tpad;
+    const int *main_emit, *stag, *lvl0, *fl;
+    int nstag, nlvl0, stag_hist_nelts;
+    const int *stag_begin;    // (dd+1): staggered entries of pass r are [stag_begin[r], stag_begin[r+1])
+    int M, pf_rank;
+    float *state;             // per (global beam, j): pass history + staggered history
+    long state_bstride;
+    int B, amb, dd, Tp, beam0, amb1_dd1;   // amb1_dd1: stage-1 dd rank (ring row = fc*2^amb1_dd1 + j)
+    long chunk;
+};
+long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts);
+int stage2_lds_bytes(int dd, int stag_hist_nelts);
+void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st);
+
+
+// ---- peak finder over (B, ndm_out, M, rowlen) rows -> out_max, out_argmax ----
+struct PfArgs {
+    void *sb;                 // rows [tpad history | Tp | future (max final lag)]
+    long sb_bstride;
+    int rowlen, tpad, flmax;
+    const int *m_to_n;        // (M)
+    const float *wt;          // (B, ndm_wt, nt_wt, P, N) non-negative weights
+    long wt_bstride;
+    void *out_max;            // (B, ndm_out, nt_out), storage dtype
+    unsigned int *out_argmax; // (B, ndm_out, nt_out)
+    long out_bstride;
+    int B, ndm_out, M, N, P, nlev, Tp, Dout, Dcore, ndm_wt, nt_wt, nt_out;
+    int beam0;                // global index of the batch's first beam (sb rows are per global beam)
+};
+int pf_lds_bytes(int M, int tpad, int nlev, int Dcore);
+void launch_pf(Dtype dt, const PfArgs &a, hipStream_t st);
+
+}  // namespace pirate_amd
+
+#endif

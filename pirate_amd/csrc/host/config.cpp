@@ -44,6 +44,10 @@ Dtype dtype_from_str(const std::string &s)
 static std::string fmt_double(double x)
 {
     char buf[64];
+    if (x == std::floor(x) && std::fabs(x) < 1.0e15) {
+        snprintf(buf, sizeof(buf), "%.0f", x);
+        return buf;
+    }
     for (int prec = 1; prec <= 17; prec++) {
         snprintf(buf, sizeof(buf), "%.*g", prec, x);
         if (std::strtod(buf, nullptr) == x)

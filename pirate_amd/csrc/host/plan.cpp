@@ -11,6 +11,10 @@ namespace pirate_amd {
 static std::string fmt_g(double x)
 {
     char buf[64];
+    if (x == std::floor(x) && std::fabs(x) < 1.0e15) {
+        snprintf(buf, sizeof(buf), "%.0f", x);
+        return buf;
+    }
     for (int prec = 1; prec <= 17; prec++) {
         snprintf(buf, sizeof(buf), "%.*g", prec, x);
         if (std::strtod(buf, nullptr) == x)

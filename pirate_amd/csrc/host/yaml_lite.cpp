@@ -334,7 +334,25 @@ NodeP YamlNode::parse(const std::string &text, const std::string &name)
         int ind = 0;
         while (ind < int(s.size()) && s[ind] == ' ')
             ind++;
-        bp.lines.push_back({ind, s.substr(ind), lineno});
+        // A flow collection may continue over several lines: join until brackets balance.
+        std::string body = s.substr(ind);
+        auto depth_of = [](const std::string &x) {
+            int d = 0;
+            char q = 0;
+            for (char c : x) {
+                if (q) { if (c == q) q = 0; continue; }
+                if (c == '\'' || c == '"') q = c;
+                else if (c == '[' || c == '{') d++;
+                else if (c == ']' || c == '}') d--;
+            }
+            return d;
+        };
+        int start_line = lineno;
+        while (depth_of(body) > 0 && std::getline(in, raw)) {
+            lineno++;
+            body += " " + trim(rtrim(strip_comment(raw)));
+        }
+        bp.lines.push_back({ind, body, start_line});
     }
 
     if (bp.lines.empty()) {

@@ -1,0 +1,203 @@
+"""GpuDedisperser: Python surface of the reference's GpuDedisperser
+(src_pybind11/pirate_pybind11.cpp:453-594, pirate_frb/GpuDedisperser.py:105-197),
+backed by the C-ABI of libpirate_amd.so. Arrays are exposed as torch tensors that view
+device memory owned by the dedisperser (torch on ROCm is used for streams and views).
+"""
+
+import ctypes
+from contextlib import contextmanager
+
+import numpy as np
+
+from . import _lib as L
+
+_TYPESTR = {'float32': '<f4', 'float16': '<f2', 'bfloat16': '<V2'}
+
+
+class _DeviceView:
+    """Minimal __cuda_array_interface__ provider for torch.as_tensor()."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {'shape': tuple(int(s) for s in shape), 'typestr': typestr,
+                                         'data': (int(ptr), False), 'version': 2, 'strides': None}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def device_view(ptr, shape, dtype):
+    """torch tensor viewing 'ptr' (no copy). dtype in {'float32','float16','bfloat16','uint32'}."""
+    torch = _torch()
+    if dtype == 'uint32':
+        t = torch.as_tensor(_DeviceView(ptr, shape, '<i4'), device='cuda')
+        return t   # int32 view of the uint32 tokens (same bits)
+    if dtype == 'bfloat16':
+        t = torch.as_tensor(_DeviceView(ptr, shape, '<i2'), device='cuda')
+        return t.view(torch.bfloat16)
+    return torch.as_tensor(_DeviceView(ptr, shape, _TYPESTR[dtype]), device='cuda')
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return _torch().cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class GpuDedisperserOutputs:
+    """Outputs of one beam batch: out_max[itree], out_argmax[itree] (tensor views valid until
+    the matching release). Reference: pirate_frb/core/GpuDedisperserOutputs.py."""
+
+    def __init__(self, out_max, out_argmax, ichunk_zero_based, ichunk_fpga_based, ibeam):
+        self.out_max = out_max
+        self.out_argmax = out_argmax
+        self.ichunk_zero_based = ichunk_zero_based
+        self.ichunk_fpga_based = ichunk_fpga_based
+        self.ibeam = ibeam
+
+
+class GpuDedisperser:
+    """MI355X dedisperser with the reference's acquire/release interface.
+
+    Example::
+
+        dd = GpuDedisperser(plan, cuda_device_id=0)
+        dd.allocate()
+        dd.fill_all_weights(itree, w)           # per tree
+        for seq_id in range(nchunks * dd.nbatches):
+            with dd.get_input(seq_id) as arr:    # (beams_per_batch, nfreq, nt_in) tensor
+                arr.copy_(data)
+            with dd.get_output(seq_id) as out:
+                out.out_max[itree], out.out_argmax[itree]
+    """
+
+    def __init__(self, plan, stream_pool=None, cuda_device_id=0, num_consumers=1, nbatches_out=0, nbatches_wt=0,
+                 initial_chunk=0, synchronous=False):
+        self.plan = plan
+        self.config = plan.config
+        self.dtype = plan.dtype
+        self.nfreq = plan.nfreq
+        self.nt_in = plan.nt_in
+        self.total_beams = plan.beams_per_gpu
+        self.beams_per_batch = plan.beams_per_batch
+        self.nstreams = plan.num_active_batches
+        self.nbatches = self.total_beams // self.beams_per_batch
+        self.ntrees = plan.ntrees
+        self.trees = plan.trees
+        self.num_consumers = num_consumers if num_consumers > 0 else 1
+        self.nbatches_out = nbatches_out if nbatches_out > 0 else self.nstreams
+        self.nbatches_wt = nbatches_wt if nbatches_wt > 0 else self.nbatches_out
+        self.initial_chunk = initial_chunk
+        self.cuda_device_id = cuda_device_id
+        p = L.DedisperserParams(nbatches_out, nbatches_wt, self.num_consumers, cuda_device_id, int(synchronous),
+                                initial_chunk)
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().pdd_dedisperser_create(plan._h, ctypes.byref(p), ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h and L._lib is not None:
+            L._lib.pdd_dedisperser_destroy(h)
+            self._h = None
+
+    @property
+    def Dcore(self):
+        return [t.Dcore for t in self.trees]
+
+    def allocate(self, gpu_allocator=None, host_allocator=None):
+        L.check(L.lib().pdd_dedisperser_allocate(self._h))
+
+    def gpu_bytes(self):
+        n = ctypes.c_int64()
+        L.check(L.lib().pdd_dedisperser_gpu_bytes(self._h, ctypes.byref(n)))
+        return n.value
+
+    # ---- low-level interface (reference: _acquire_input etc.) ----
+
+    def _acquire_input(self, seq_id, stream_ptr):
+        slot = ctypes.c_void_p()
+        L.check(L.lib().pdd_acquire_input(self._h, seq_id, ctypes.c_void_p(stream_ptr), ctypes.byref(slot)))
+        return device_view(slot.value, (self.beams_per_batch, self.nfreq, self.nt_in), self.dtype)
+
+    def _release_input_and_launch_dd_kernels(self, seq_id, stream_ptr):
+        L.check(L.lib().pdd_release_input_and_launch(self._h, seq_id, ctypes.c_void_p(stream_ptr)))
+
+    def _acquire_output(self, consumer_id, seq_id, stream_ptr, sync=False):
+        n = self.ntrees
+        om = (ctypes.c_void_p * n)()
+        oa = (ctypes.c_void_p * n)()
+        L.check(L.lib().pdd_acquire_output(self._h, consumer_id, seq_id, ctypes.c_void_p(stream_ptr), int(sync),
+                                           om, oa, n))
+        mx, am = [], []
+        for i, t in enumerate(self.trees):
+            shape = (self.beams_per_batch, t.ndm_out, t.nt_out)
+            mx.append(device_view(om[i], shape, self.dtype))
+            am.append(device_view(oa[i], shape, 'uint32'))
+        ichunk = seq_id // self.nbatches
+        return GpuDedisperserOutputs(mx, am, ichunk, ichunk + self.initial_chunk,
+                                     (seq_id % self.nbatches) * self.beams_per_batch)
+
+    def _release_output(self, consumer_id, seq_id, stream_ptr):
+        L.check(L.lib().pdd_release_output(self._h, consumer_id, seq_id, ctypes.c_void_p(stream_ptr)))
+
+    # ---- context managers (reference: pirate_frb/GpuDedisperser.py:105-197) ----
+
+    @contextmanager
+    def get_input(self, seq_id, stream=None):
+        sp = _stream_ptr(stream)
+        arr = self._acquire_input(seq_id, sp)
+        try:
+            yield arr
+        finally:
+            self._release_input_and_launch_dd_kernels(seq_id, sp)
+
+    @contextmanager
+    def get_output(self, seq_id, stream=None, consumer_id=0):
+        sp = _stream_ptr(stream)
+        out = self._acquire_output(consumer_id, seq_id, sp)
+        try:
+            yield out
+        finally:
+            self._release_output(consumer_id, seq_id, sp)
+
+    # ---- weights ----
+
+    def wt_shape(self, itree):
+        t = self.trees[itree]
+        return (self.nbatches_wt, self.beams_per_batch, t.ndm_wt, t.nt_wt, t.nprofiles, t.nsubbands_N)
+
+    def fill_all_weights(self, itree, weights):
+        """weights: (nbatches_wt, beams_per_batch, ndm_wt, nt_wt, nprofiles, N) float, >= 0."""
+        w = weights
+        if hasattr(w, 'detach'):
+            w = w.detach().cpu().numpy()
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.shape != self.wt_shape(itree):
+            raise ValueError(f'fill_all_weights: expected shape {self.wt_shape(itree)}, got {w.shape}')
+        L.check(L.lib().pdd_fill_all_weights(self._h, itree, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size))
+
+    def stop(self, reason=''):
+        L.check(L.lib().pdd_stop(self._h, reason.encode()))
+
+    def set_kernel_timing(self, enable):
+        L.check(L.lib().pdd_kernel_timing(self._h, int(enable)))
+
+    def kernel_timing(self):
+        """{kernel name: (launches, total_ms)} accumulated since set_kernel_timing(True)."""
+        rep = L.get_string(L.lib().pdd_kernel_timing_report, self._h)
+        out = {}
+        for line in rep.splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
+
+    def debug_copy(self, name):
+        """Host copy of an internal buffer (tests): 'grid', 'lds<p>', 'sb<itree>', 'ring<p>'."""
+        n = ctypes.c_size_t()
+        L.check(L.lib().pdd_debug_copy(self._h, name.encode(), None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        L.check(L.lib().pdd_debug_copy(self._h, name.encode(), buf.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)))
+        return buf
