@@ -123,6 +123,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--timing-steps', type=int, default=4)
+    ap.add_argument('--calib-copy', action='store_true',
+                    help='run one 1 GiB device-to-device copy first (reference dispatch for HBM counters)')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,6 +135,14 @@ def main():
     import torch
     import pirate_amd as pa
     from gpu_cases import CFG2
+
+    if args.calib_copy:
+        src = torch.ones(2**28, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print('[bench] calibration copy: 1 GiB read + 1 GiB write', file=sys.stderr, flush=True)
 
     cfg = pa.DedispersionConfig.from_dict(CFG2)
     plan = pa.DedispersionPlan(cfg)

@@ -7,8 +7,6 @@
 
 namespace pirate_amd {
 
-bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp);   // k_pf.hip
-
 #define HIP_CHECK(expr)                                                                   \
     do {                                                                                  \
         hipError_t _e = (expr);                                                           \
@@ -158,7 +156,7 @@ void GpuDedisperser::build_tables()
 
         T.tpad = int(std::max(2 * t.pf.max_width, 4L));
         T.nlev = std::max(integer_log2(t.pf.max_width), 1);
-        T.rowlen = T.tpad + t.nt_ds + T.flmax;
+        T.rowlen = int(align_up(T.tpad + t.nt_ds + T.flmax, 4));   // 16-byte aligned rows (vector loads)
     }
 }
 

@@ -24,6 +24,26 @@ __device__ __forceinline__ void st_st(float *p, float x) { *p = x; }
 __device__ __forceinline__ void st_st(__half *p, float x) { *p = __float2half(x); }
 __device__ __forceinline__ void st_st(__hip_bfloat16 *p, float x) { *p = __float2bfloat16(x); }
 
+// Four consecutive storage elements (16-byte aligned for float, 8-byte for 16-bit types).
+__device__ __forceinline__ float4 ld4_st(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+
+__device__ __forceinline__ float4 ld4_st(const __half *p)
+{
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return make_float4(__half2float(__ushort_as_half((unsigned short) (u.x & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short) (u.x >> 16))),
+                       __half2float(__ushort_as_half((unsigned short) (u.y & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short) (u.y >> 16))));
+}
+
+__device__ __forceinline__ float4 ld4_st(const __hip_bfloat16 *p)
+{
+    // bfloat16 -> float is exact: the 16 bits are the high half of the float.
+    const uint2 u = *reinterpret_cast<const uint2 *>(p);
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+}
+
 __device__ __forceinline__ int bitrev_dev(int i, int nbits)
 {
     return (nbits > 0) ? int(__brev(unsigned(i)) >> (32 - nbits)) : 0;

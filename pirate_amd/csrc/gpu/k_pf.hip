@@ -6,251 +6,311 @@
 //             dt = min(Dcore, 2^l); with x_k = B_l(e - (3-k)*2^l):
 //               p=0 (l=0): x3      p=3l+1: x2+x3      p=3l+2: (x1/2 + x2) + x3/2
 //               p=3l+3: ((x0/2 + x1) + x2) + x3/2
-//   out_max[tout] = max over (l, m, isamp, p) of w[p, n(m)] * y,  first maximum in the loop
+//   out_max[tout] = max over (l, m, isamp, p) of w[p, n(m)] * y; the first maximum in the loop
 //   order (l, m, isamp, p) wins ties; token = t | (p << 8) | (m << 16), t = isamp*dt.
 //
-// Algorithm (bit-exact with the float32 reference, for weights >= 0):
-//   1. Per tile, the M streams of one DM row are staged in LDS and every boxcar level is
-//      built there with the same binary-tree association as the reference.
-//   2. Group maxima G(l,m,bin) = max_p w_p * max_isamp y_p. Rounding is monotonic, so for
-//      w_p >= 0 this equals the maximum of the individually rounded products w_p*y.
-//   3. Per bin, the first (l, m) in loop order with the largest G is the winning group;
-//      a short scan of that group in (isamp, p) order finds the first candidate whose
-//      rounded product equals the maximum: that candidate is the reference's argmax.
+// Mapping: one lane per stream (DM row, multiplet m); a block holds 256/M DM rows. Each lane
+// walks its stream through time once. Every boxcar level keeps its last values in a register
+// window (Wmax and Dout are template parameters, so every shift, sum and candidate of one
+// output bin is unrolled at compile time). Each candidate is weighted as it is formed and
+// compared with the level's running best (strict '>', candidates visited in (isamp, p)
+// order), so the lane ends each bin with the exact first maximum per level. Levels are then
+// combined in order (strict '>'), and the multiplets of a DM row are reduced through LDS in
+// (level, m) order: the result is the reference's maximum and argmax token, bit for bit.
 //
-// Rows are [tpad history | Tp samples | future] (see the stage-2 kernel); at the end of a
-// chunk the history and future regions are rotated into place for the next chunk.
+// Samples are staged through LDS in tiles of TL positions (coalesced 4-element loads);
+// tiles are aligned to output bins. Positions before the row start are read as zeros,
+// which equals the zero initial state of the windows.
+//
+// Rows are [tpad history | Tp samples | future (final lags)]; at the end of a chunk the
+// history and future regions are rotated into place for the next chunk.
 
 #include "device_common.hpp"
 #include "kernels.hpp"
 
-#include <climits>
-
 namespace pirate_amd {
 
 static constexpr int kPfThreads = 256;
-static constexpr int kPfMaxLev = 6;
+static constexpr int kPfMaxDout = 32;
 
-struct PfGeom {
-    int TT;                      // tile length (PF samples)
-    int nbins;                   // TT / Dout
-    int n[kPfMaxLev];            // entries per level
-    int off[kPfMaxLev];          // offset of level l within a stream's level block
-    int per_stream;              // floats per stream
+bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp)
+{
+    (void) nlev;
+    if (M < 1 || M > kPfThreads || Dout < 1 || Dout > kPfMaxDout || (Dout & (Dout - 1)))
+        return false;
+    if (tpad % 4)
+        return false;
+    const int dc = (Dout < 8) ? Dout : 8;
+    return (Dcore == dc) && (Tp % 32 == 0);
+}
+
+template<int LW, int LD>
+struct PfShape {
+    static constexpr int W = 1 << LW;
+    static constexpr int NLEV = (LW > 1) ? LW : 1;
+    static constexpr int P = 1 + 3 * LW;
+    static constexpr int DOUT = 1 << LD;
+    static constexpr int DCORE = (DOUT < 8) ? DOUT : 8;
+    static constexpr int TL = (DOUT >= 4) ? 32 : 8 * DOUT;       // tile length (positions)
+    static constexpr int NB = TL / DOUT;                          // output bins per tile
+    static constexpr int dt(int l) { return ((1 << l) < DCORE) ? (1 << l) : DCORE; }
+    static constexpr int sgrid(int l) { return (1 << l) / dt(l); }   // grid points per 2^l samples
+    static constexpr int win(int l) { return 3 * sgrid(l) + 1; }
+    static constexpr int woff(int l) { return (l == 0) ? 0 : woff(l - 1) + win(l - 1); }
+    static constexpr int WTOT = woff(NLEV);
+    // Occupancy target: 4 waves/SIMD (128 VGPRs) when the register windows are small,
+    // else 2 (the compiler's unrolled schedule would otherwise spill).
+    static constexpr int WAVES = (WTOT + P <= 32 && LD >= 2) ? 4 : 2;
 };
 
-__host__ __device__ inline PfGeom pf_geom(int TT, int tpad, int nlev, int Dcore, int Dout)
-{
-    PfGeom g;
-    g.TT = TT;
-    g.nbins = TT / Dout;
-    int o = 0;
-    for (int l = 0; l < kPfMaxLev; l++) {
-        if (l < nlev) {
-            int w = 1 << l, dt = (Dcore < w) ? Dcore : w;
-            g.n[l] = (TT + tpad - w) / dt + 1;
-            g.off[l] = o;
-            o += g.n[l];
-        }
-        else {
-            g.n[l] = 0;
-            g.off[l] = o;
+template<typename S>
+struct PfLane {
+    float w[S::WTOT];          // per level l: w[woff(l) + i] = B_l(e - i*dt(l)), e = newest end
+    float bv[S::NLEV];         // per level: best weighted value in the current bin
+    unsigned int bc[S::NLEV];  // per level: its t | (p << 8)
+    float wt[S::P];            // weights of the current bin, per profile
+
+    template<int L>
+    __device__ __forceinline__ void push(float v)
+    {
+        constexpr int o = S::woff(L), n = S::win(L);
+#pragma unroll
+        for (int i = n - 1; i > 0; i--)
+            w[o + i] = w[o + i - 1];
+        w[o] = v;
+    }
+
+    template<int L, bool FIRST>
+    __device__ __forceinline__ void upd(float v, unsigned int code)
+    {
+        if constexpr (FIRST) {
+            bv[L] = v;
+            bc[L] = code;
+        } else {
+            const bool g = v > bv[L];
+            bv[L] = g ? v : bv[L];
+            bc[L] = g ? code : bc[L];
         }
     }
-    g.per_stream = o;
-    return g;
-}
 
-static int pf_pick_tt(int M, int tpad, int nlev, int Dcore, int Dout, int Tp)
-{
-    int best = 0;
-    for (int TT = Dout; TT <= Tp && TT <= 512; TT *= 2) {
-        if (Tp % TT)
-            continue;
-        PfGeom g = pf_geom(TT, tpad, nlev, Dcore, Dout);
-        long bytes = sizeof(float) * (long(M) * g.per_stream + long(g.nbins) * nlev * M) + 8 * g.nbins;
-        if (bytes <= 96 * 1024)
-            best = TT;
+    template<int L, int POS>
+    __device__ __forceinline__ void candidates()
+    {
+        constexpr int o = S::woff(L), s = S::sgrid(L);
+        constexpr unsigned int t = unsigned(POS + 1 - S::dt(L));
+        constexpr bool first = (t == 0);
+        const float x3 = w[o], x2 = w[o + s], x1 = w[o + 2 * s], x0 = w[o + 3 * s];
+        if constexpr (L == 0)
+            upd<0, first>(wt[0] * x3, t);
+        if constexpr (S::P > 1) {
+            constexpr int p = 3 * L + 1;
+            const float h3 = 0.5f * x3;
+            const float y1 = x2 + x3;
+            const float y2 = (0.5f * x1 + x2) + h3;
+            const float y3 = ((0.5f * x0 + x1) + x2) + h3;
+            upd<L, first && (L > 0)>(wt[p] * y1, t | (unsigned(p) << 8));
+            upd<L, false>(wt[p + 1] * y2, t | (unsigned(p + 1) << 8));
+            upd<L, false>(wt[p + 2] * y3, t | (unsigned(p + 2) << 8));
+        }
     }
-    return best;
+
+    template<int L, int POS, bool EVAL>
+    __device__ __forceinline__ void levels()
+    {
+        if constexpr (L < S::NLEV) {
+            if constexpr ((POS + 1) % S::dt(L) == 0) {
+                if constexpr (L > 0) {
+                    constexpr int op = S::woff(L - 1), back = S::sgrid(L - 1);
+                    push<L>(w[op + back] + w[op]);
+                }
+                if constexpr (EVAL)
+                    candidates<L, POS>();
+                levels<L + 1, POS, EVAL>();
+            }
+        }
+    }
+
+    // One output bin (DOUT positions); xs[POS] is the sample at in-bin position POS.
+    template<bool EVAL, int POS = 0>
+    __device__ __forceinline__ void bin(const float *xs)
+    {
+        if constexpr (POS < S::DOUT) {
+            push<0>(xs[POS]);
+            levels<0, POS, EVAL>();
+            bin<EVAL, POS + 1>(xs);
+        }
+    }
+
+    __device__ __forceinline__ void combine(float &V, unsigned int &C) const
+    {
+        V = bv[0];
+        C = bc[0];
+#pragma unroll
+        for (int l = 1; l < S::NLEV; l++) {
+            const bool g = bv[l] > V;
+            V = g ? bv[l] : V;
+            C = g ? bc[l] : C;
+        }
+    }
+};
+
+__device__ __forceinline__ int pf_level_of_token(unsigned int tok)
+{
+    const int p = int((tok >> 8) & 0xffu);
+    return (p == 0) ? 0 : (p - 1) / 3;
 }
 
-int pf_lds_bytes(int M, int tpad, int nlev, int Dcore)
+template<typename Tst, int LW, int LD>
+__global__ void __launch_bounds__(kPfThreads) __attribute__((amdgpu_waves_per_eu(PfShape<LW, LD>::WAVES)))
+k_pf(PfArgs a)
 {
-    (void) Dcore;
-    return int(sizeof(float) * long(M) * 4 * tpad);   // nominal; the kernel picks its own tile
-}
+    using S = PfShape<LW, LD>;
+    constexpr int TL = S::TL, NB = S::NB, STR = TL + 1, VPR = TL / 4;
+    __shared__ float tile[kPfThreads * STR];
+    __shared__ float rv[kPfThreads * NB];
+    __shared__ unsigned int rt[kPfThreads * NB];
 
-// Level-l boxcar ending at PF time e, read from the stream's level block in LDS.
-__device__ __forceinline__ float lv_at(const float *blk, const PfGeom &g, int l, int e, int tile_t0, int tpad, int Dcore)
-{
-    const int w = 1 << l, dt = (Dcore < w) ? Dcore : w;
-    const int i = (e - (tile_t0 - tpad + w)) / dt;
-    return blk[g.off[l] + i];
-}
+    const int tid = threadIdx.x;
+    const int M = a.M, N = a.N;
+    const int ndpf = kPfThreads / M;
+    const int dpf0 = blockIdx.x * ndpf;
+    const int b = blockIdx.y;
+    const int nd = min(ndpf, a.ndm_out - dpf0);
+    const int nstreams = nd * M;
+    const bool active = tid < nstreams;
+    const int dl = tid / M, m = tid - dl * M;
 
-template<typename Tst>
-__global__ void __launch_bounds__(kPfThreads)
-k_pf(PfArgs a, PfGeom g)
-{
-    extern __shared__ float smem[];
-    const int M = a.M, nlev = a.nlev, P = a.P, N = a.N;
-    float *lv = smem;                                        // [M][per_stream]
-    float *Gv = lv + M * g.per_stream;                       // [nbins][nlev][M]
-    float *bval = Gv + g.nbins * nlev * M;                   // [nbins]
-    int *bkey = reinterpret_cast<int *>(bval + g.nbins);     // [nbins]
-
-    const int dpf = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-    Tst *rows = reinterpret_cast<Tst *>(a.sb) + long(a.beam0 + b) * a.sb_bstride + long(dpf) * M * a.rowlen;
-    const float *wtb = a.wt + long(b) * a.wt_bstride;
+    Tst *rows = reinterpret_cast<Tst *>(a.sb) + long(a.beam0 + b) * a.sb_bstride + long(dpf0) * M * a.rowlen;
     const int Wds = a.ndm_out / a.ndm_wt;
     const int Tds = a.nt_out / a.nt_wt;
-    const float *wrow_d = wtb + long(dpf / Wds) * a.nt_wt * P * N;
+    const float *wrow = a.wt + long(b) * a.wt_bstride + long(active ? (dpf0 + dl) / Wds : 0) * a.nt_wt * S::P * N
+                        + (active ? a.m_to_n[m] : 0);
+    Tst *outm = reinterpret_cast<Tst *>(a.out_max) + long(b) * a.out_bstride;
+    unsigned int *outa = a.out_argmax + long(b) * a.out_bstride;
 
-    for (int t0 = 0; t0 < a.Tp; t0 += g.TT) {
-        // ---- stage level 0: samples at PF times [t0 - tpad, t0 + TT) = row positions [t0, t0 + TT + tpad)
-        const int n0 = g.n[0];
-        for (int idx = tid; idx < M * n0; idx += kPfThreads) {
-            const int m = idx / n0, i = idx % n0;
-            lv[m * g.per_stream + i] = ld_st(rows + long(m) * a.rowlen + t0 + i);
+    PfLane<S> ln;
+#pragma unroll
+    for (int i = 0; i < S::WTOT; i++)
+        ln.w[i] = 0.0f;
+
+    const int nwarm = (a.tpad + TL - 1) / TL;
+    const int ntiles = nwarm + a.Tp / TL;
+    const float *xs = tile + tid * STR;
+
+    for (int it = 0; it < ntiles; it++) {
+        const int p0 = a.tpad + (it - nwarm) * TL;
+
+        // ---- stage [nstreams][TL] samples (4 per load, VPR loads per row segment)
+#pragma unroll
+        for (int k = 0; k < VPR; k++) {
+            const int idx = tid + k * kPfThreads;
+            const int s = idx / VPR, i = (idx - s * VPR) * 4;
+            if (s < nstreams) {
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (p0 + i >= 0)
+                    v = ld4_st(rows + long(s) * a.rowlen + p0 + i);
+                float *d = tile + s * STR + i;
+                d[0] = v.x;
+                d[1] = v.y;
+                d[2] = v.z;
+                d[3] = v.w;
+            }
         }
         __syncthreads();
 
-        // ---- build boxcar levels: B_l(e) = B_{l-1}(e - 2^(l-1)) + B_{l-1}(e)
-        for (int l = 1; l < nlev; l++) {
-            const int w = 1 << l, dt = min(a.Dcore, w);
-            const int wp = w >> 1, dtp = min(a.Dcore, wp);
-            const int nl = g.n[l];
-            for (int idx = tid; idx < M * nl; idx += kPfThreads) {
-                const int m = idx / nl, i = idx % nl;
-                float *blk = lv + m * g.per_stream;
-                const int e = t0 - a.tpad + w + i * dt;
-                const int i1 = (e - wp - (t0 - a.tpad + wp)) / dtp;
-                const int i2 = (e - (t0 - a.tpad + wp)) / dtp;
-                blk[g.off[l] + i] = blk[g.off[l - 1] + i1] + blk[g.off[l - 1] + i2];
+        if (it < nwarm) {
+            if (active) {
+#pragma unroll 1
+                for (int j = 0; j < NB; j++)
+                    ln.template bin<false>(xs + j * S::DOUT);
             }
-            __syncthreads();
-        }
-
-        // ---- group maxima G(l, m, bin)
-        const int nitems = g.nbins * nlev * M;
-        for (int idx = tid; idx < nitems; idx += kPfThreads) {
-            const int m = idx % M;
-            const int l = (idx / M) % nlev;
-            const int bin = idx / (M * nlev);
-            const int tout = t0 / a.Dout + bin;
-            const int n = a.m_to_n[m];
-            const float *wp = wrow_d + long(tout / Tds) * P * N;
-            const float *blk = lv + m * g.per_stream;
-            const int w = 1 << l, dt = min(a.Dcore, w);
-            const int nsamp = a.Dout / dt;
-            float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
-            for (int is = 0; is < nsamp; is++) {
-                const int e = tout * a.Dout + (is + 1) * dt;
-                const float x3 = lv_at(blk, g, l, e, t0, a.tpad, a.Dcore);
-                if (l == 0)
-                    m0 = fmaxf(m0, x3);
-                if (P > 1) {
-                    const float x2 = lv_at(blk, g, l, e - w, t0, a.tpad, a.Dcore);
-                    const float x1 = lv_at(blk, g, l, e - 2 * w, t0, a.tpad, a.Dcore);
-                    const float x0 = lv_at(blk, g, l, e - 3 * w, t0, a.tpad, a.Dcore);
-                    const float y1 = x2 + x3;
-                    const float y2 = (0.5f * x1 + x2) + 0.5f * x3;
-                    const float y3 = ((0.5f * x0 + x1) + x2) + 0.5f * x3;
-                    m1 = fmaxf(m1, y1);
-                    m2 = fmaxf(m2, y2);
-                    m3 = fmaxf(m3, y3);
+        } else {
+            const int tb0 = (it - nwarm) * NB;
+            if (active) {
+#pragma unroll 1
+                for (int j = 0; j < NB; j++) {
+                    const float *wp = wrow + long((tb0 + j) / Tds) * S::P * N;
+#pragma unroll
+                    for (int p = 0; p < S::P; p++)
+                        ln.wt[p] = wp[p * N];
+                    ln.template bin<true>(xs + j * S::DOUT);
+                    float V;
+                    unsigned int C;
+                    ln.combine(V, C);
+                    rv[tid * NB + j] = V;
+                    rt[tid * NB + j] = C | (unsigned(m) << 16);
                 }
             }
-            float G = -INFINITY;
-            if (l == 0)
-                G = fmaxf(G, wp[0 * N + n] * m0);
-            if (P > 1) {
-                G = fmaxf(G, wp[(3 * l + 1) * N + n] * m1);
-                G = fmaxf(G, wp[(3 * l + 2) * N + n] * m2);
-                G = fmaxf(G, wp[(3 * l + 3) * N + n] * m3);
-            }
-            Gv[(bin * nlev + l) * M + m] = G;
         }
         __syncthreads();
 
-        // ---- per bin: winning group (first (l, m) in loop order), then the winning candidate
-        for (int bin = tid; bin < g.nbins; bin += kPfThreads) {
-            float best = -1.0e30f;
-            int key = -1;
-            for (int l = 0; l < nlev; l++)
-                for (int m = 0; m < M; m++) {
-                    const float v = Gv[(bin * nlev + l) * M + m];
-                    if (v > best) {
-                        best = v;
-                        key = l * M + m;
+        // ---- reduce over multiplets: first (level, m) in loop order with the largest value
+        if (it >= nwarm) {
+            const int tb0 = (it - nwarm) * NB;
+            for (int idx = tid; idx < nd * NB; idx += kPfThreads) {
+                const int d = idx / NB, j = idx - d * NB;
+                const int s0 = d * M;
+                float bv = rv[s0 * NB + j];
+                unsigned int bt = rt[s0 * NB + j];
+                int bk = pf_level_of_token(bt) * M;
+                for (int mm = 1; mm < M; mm++) {
+                    const float v = rv[(s0 + mm) * NB + j];
+                    const unsigned int tk = rt[(s0 + mm) * NB + j];
+                    const int key = pf_level_of_token(tk) * M + mm;
+                    if (v > bv || (v == bv && key < bk)) {
+                        bv = v;
+                        bt = tk;
+                        bk = key;
                     }
                 }
-
-            const int tout = t0 / a.Dout + bin;
-            unsigned int token = ~0u;
-            float outv = -1.0e30f;
-            if (key >= 0) {
-                const int l = key / M, m = key % M;
-                const int n = a.m_to_n[m];
-                const float *wp = wrow_d + long(tout / Tds) * P * N;
-                const float *blk = lv + m * g.per_stream;
-                const int w = 1 << l, dt = min(a.Dcore, w);
-                const int nsamp = a.Dout / dt;
-                bool found = false;
-                for (int is = 0; is < nsamp && !found; is++) {
-                    const int e = tout * a.Dout + (is + 1) * dt;
-                    const float x3 = lv_at(blk, g, l, e, t0, a.tpad, a.Dcore);
-                    const unsigned int tk0 = (unsigned(m) << 16) | unsigned(is * dt);
-                    if (l == 0) {
-                        const float v = wp[0 * N + n] * x3;
-                        if (v == best) { found = true; token = tk0; outv = v; }
-                    }
-                    if (!found && P > 1) {
-                        const float x2 = lv_at(blk, g, l, e - w, t0, a.tpad, a.Dcore);
-                        const float x1 = lv_at(blk, g, l, e - 2 * w, t0, a.tpad, a.Dcore);
-                        const float x0 = lv_at(blk, g, l, e - 3 * w, t0, a.tpad, a.Dcore);
-                        const float y[3] = {x2 + x3, (0.5f * x1 + x2) + 0.5f * x3, ((0.5f * x0 + x1) + x2) + 0.5f * x3};
-                        for (int q = 0; q < 3 && !found; q++) {
-                            const int p = 3 * l + 1 + q;
-                            const float v = wp[p * N + n] * y[q];
-                            if (v == best) { found = true; token = tk0 | (unsigned(p) << 8); outv = v; }
-                        }
-                    }
-                }
+                const long o = long(dpf0 + d) * a.nt_out + tb0 + j;
+                st_st(outm + o, bv);
+                outa[o] = bt;
             }
-            Tst *om = reinterpret_cast<Tst *>(a.out_max) + long(b) * a.out_bstride + long(dpf) * a.nt_out + tout;
-            st_st(om, outv);
-            a.out_argmax[long(b) * a.out_bstride + long(dpf) * a.nt_out + tout] = token;
         }
-        __syncthreads();
     }
 
-    // ---- rotate each row for the next chunk: history <- last tpad samples, future -> start
-    for (int idx = tid; idx < M * (a.tpad + a.flmax); idx += kPfThreads) {
-        const int m = idx / (a.tpad + a.flmax), i = idx % (a.tpad + a.flmax);
-        Tst *row = rows + long(m) * a.rowlen;
-        // positions [Tp, Tp + tpad + flmax) -> [0, tpad + flmax); requires Tp >= tpad + flmax
+    // ---- rotate rows for the next chunk: positions [Tp, Tp + tpad + flmax) -> [0, ...)
+    // (all staging loads completed before the last barrier; Tp >= tpad + flmax is checked
+    // on the host, so no position is both read and written here)
+    const int nrot = a.tpad + a.flmax;
+    for (int idx = tid; idx < nstreams * nrot; idx += kPfThreads) {
+        const int s = idx / nrot, i = idx - s * nrot;
+        Tst *row = rows + long(s) * a.rowlen;
         const Tst v = row[a.Tp + i];
         row[i] = v;
+    }
+}
+
+template<typename Tst, int LW>
+static void launch_pf_lw(const PfArgs &a, hipStream_t st, dim3 grid)
+{
+    dim3 block(kPfThreads);
+    switch (a.Dout) {
+#define PDD_PF(LDV)                                                          \
+        case (1 << LDV):                                                     \
+            hipLaunchKernelGGL((k_pf<Tst, LW, LDV>), grid, block, 0, st, a); \
+            break;
+        PDD_PF(0) PDD_PF(1) PDD_PF(2) PDD_PF(3) PDD_PF(4) PDD_PF(5)
+#undef PDD_PF
+        default: break;
     }
 }
 
 template<typename Tst>
 static void launch_pf_t(const PfArgs &a, hipStream_t st)
 {
-    const int TT = pf_pick_tt(a.M, a.tpad, a.nlev, a.Dcore, a.Dout, a.Tp);
-    PfGeom g = pf_geom(TT, a.tpad, a.nlev, a.Dcore, a.Dout);
-    size_t shm = sizeof(float) * (size_t(a.M) * g.per_stream + size_t(g.nbins) * a.nlev * a.M) + 8 * g.nbins;
-    dim3 grid(a.ndm_out, a.B);
-    dim3 block(kPfThreads);
-    hipFuncSetAttribute((const void *) k_pf<Tst>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm));
-    hipLaunchKernelGGL(k_pf<Tst>, grid, block, shm, st, a, g);
-}
-
-bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp)
-{
-    return (nlev <= kPfMaxLev) && pf_pick_tt(M, tpad, nlev, Dcore, Dout, Tp) > 0;
+    const int ndpf = kPfThreads / a.M;
+    dim3 grid((a.ndm_out + ndpf - 1) / ndpf, a.B);
+    switch ((a.P - 1) / 3) {
+        case 0: launch_pf_lw<Tst, 0>(a, st, grid); break;
+        case 1: launch_pf_lw<Tst, 1>(a, st, grid); break;
+        case 2: launch_pf_lw<Tst, 2>(a, st, grid); break;
+        case 3: launch_pf_lw<Tst, 3>(a, st, grid); break;
+        case 4: launch_pf_lw<Tst, 4>(a, st, grid); break;
+        case 5: launch_pf_lw<Tst, 5>(a, st, grid); break;
+        default: break;
+    }
 }
 
 void launch_pf(Dtype dt, const PfArgs &a, hipStream_t st)

@@ -101,7 +101,7 @@ struct PfArgs {
     int B, ndm_out, M, N, P, nlev, Tp, Dout, Dcore, ndm_wt, nt_wt, nt_out;
     int beam0;                // global index of the batch's first beam (sb rows are per global beam)
 };
-int pf_lds_bytes(int M, int tpad, int nlev, int Dcore);
+bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp);
 void launch_pf(Dtype dt, const PfArgs &a, hipStream_t st);
 
 }  // namespace pirate_amd
