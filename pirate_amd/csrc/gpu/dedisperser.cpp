@@ -59,7 +59,7 @@ GpuDedisperser::~GpuDedisperser()
     freeit(grid_rows);
     freeit(grid_w);
     for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &stage2_lags, &sb, &d_main_emit, &d_stag,
-                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &s1_state, &s2_state, &weights, &out_max, &out_argmax})
+                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_max, &out_argmax})
         for (auto &a : *v)
             freeit(a);
     for (auto &t : timed_pending) {
@@ -144,6 +144,32 @@ void GpuDedisperser::build_tables()
         }
         T.nlvl0 = int(T.lvl0.size() / 2);
         T.stag_hist_nelts = hoff;
+
+        // Register-tree path: every stream is a final-pass output (main pairs of the last
+        // pass, or level-0 copies after the last pass), each row feeding at most one stream.
+        T.out_stream.assign(pow2(dd), -1);
+        T.fast = (T.nstag == 0);
+        for (int r = 0; r < dd && T.fast; r++)
+            for (int p = 0; p < NP; p++) {
+                const int s0 = T.main_emit[r * NP + p];
+                if (s0 < 0)
+                    continue;
+                if (r != dd - 1 || T.out_stream[p] >= 0 || T.out_stream[p + NP] >= 0)
+                    T.fast = false;
+                else {
+                    T.out_stream[p] = s0;
+                    T.out_stream[p + NP] = s0 + 1;
+                }
+            }
+        if (T.nlvl0 > 0 && dd - pf_rank != dd)
+            T.fast = false;
+        for (int k = 0; k < T.nlvl0 && T.fast; k++) {
+            const int row = T.lvl0[2 * k], s0 = T.lvl0[2 * k + 1];
+            if (T.out_stream[row] >= 0)
+                T.fast = false;
+            else
+                T.out_stream[row] = s0;
+        }
 
         for (int pf_dm = 0; pf_dm < pf_ndm; pf_dm++)
             for (int m = 0; m < M; m++) {
@@ -301,7 +327,7 @@ void GpuDedisperser::allocate()
         rings[p] = dalloc(size_t(total_beams) * ring_bstride[p] * elt_bytes);
         ring_row_off[p] = up_long(off);
         ring_row_cap[p] = up_long(cap);
-        s1_state_bstride[p] = pow2(amb1) * stage1_state_nelts_per_tree(dd1);
+        s1_state_bstride[p] = pow2(amb1) * fast_tree_state_nelts(int(dd1));
         s1_state[p] = dalloc(size_t(total_beams) * s1_state_bstride[p] * sizeof(float));
     }
 
@@ -313,6 +339,7 @@ void GpuDedisperser::allocate()
     d_fl.resize(ntrees);
     d_m_to_n.resize(ntrees);
     d_stag_begin.resize(ntrees);
+    d_out_stream.resize(ntrees);
     s2_state.resize(ntrees);
     s2_state_bstride.resize(ntrees);
     weights.resize(ntrees);
@@ -335,7 +362,9 @@ void GpuDedisperser::allocate()
         d_fl[it] = up_int(T.fl);
         d_m_to_n[it] = up_int(T.m_to_n);
         d_stag_begin[it] = up_int(T.stag_begin);
-        s2_state_bstride[it] = pow2(t.amb_rank) * stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts);
+        d_out_stream[it] = up_int(T.out_stream);
+        s2_state_bstride[it] = pow2(t.amb_rank) * (T.fast ? fast_tree_state_nelts(t.dd_rank)
+                                                          : stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts));
         s2_state[it] = dalloc(size_t(total_beams) * s2_state_bstride[it] * sizeof(float));
         weights[it] = dalloc(size_t(nbatches_wt) * B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N * sizeof(float));
         out_max[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * elt_bytes);
@@ -426,7 +455,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         hipEvent_t ev;
         const std::string nm = "stage1_p" + std::to_string(p);
         timed_begin(nm, st, ev);
-        launch_stage1(dtype, s, st);
+        launch_stage1_fast(dtype, s, st);
         timed_end(nm, st, ev);
     }
     for (long it = 0; it < ntrees; it++) {
@@ -462,10 +491,14 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.beam0 = beam0;
         s.amb1_dd1 = t.amb_rank;
         s.chunk = ichunk;
+        s.out_stream = (const int *) d_out_stream[it].ptr;
         hipEvent_t ev;
         const std::string nm2 = "stage2_t" + std::to_string(it);
         timed_begin(nm2, st, ev);
-        launch_stage2(dtype, s, st);
+        if (T.fast)
+            launch_stage2_fast(dtype, s, st);
+        else
+            launch_stage2(dtype, s, st);
         timed_end(nm2, st, ev);
 
         PfArgs f;

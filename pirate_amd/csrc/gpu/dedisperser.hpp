@@ -89,6 +89,8 @@ public:
     // Per-tree stage-2 tables (host copies, exposed for tests).
     struct TreeTables {
         std::vector<int> main_emit, stag, lvl0, fl, m_to_n, stag_begin;
+        std::vector<int> out_stream;   // register-tree path: final row -> stream (-1: none)
+        bool fast = false;             // stage 2 runs on the register-tree kernel
         int nstag = 0, nlvl0 = 0, stag_hist_nelts = 0, flmax = 0, tpad = 0, nlev = 0, rowlen = 0, M = 0, N = 0;
     };
     std::vector<TreeTables> tables;
@@ -110,7 +112,7 @@ private:
     std::vector<long> ring_bstride;
     std::vector<DeviceArray> stage2_lags;    // [itree] device int
     std::vector<DeviceArray> sb;      // [itree] (B, ndm_out, M, rowlen)
-    std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin;
+    std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin, d_out_stream;
     DeviceArray lds_state;  long lds_state_bstride = 0;
     std::vector<DeviceArray> s1_state; std::vector<long> s1_state_bstride;
     std::vector<DeviceArray> s2_state; std::vector<long> s2_state_bstride;

@@ -81,10 +81,18 @@ struct Stage2Args {
     long state_bstride;
     int B, amb, dd, Tp, beam0, amb1_dd1;   // amb1_dd1: stage-1 dd rank (ring row = fc*2^amb1_dd1 + j)
     long chunk;
+    const int *out_stream;    // register-tree path: (2^dd) final row -> stream, or -1
 };
 long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts);
 int stage2_lds_bytes(int dd, int stag_hist_nelts);
 void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st);
+
+// ---- register-resident trees (phase A / LDS delay lines / phase B), bit-identical results ----
+// Stage 1 always; stage 2 when every stream is a final-pass output (no staggered pairs, no
+// earlier emissions). State per tree instance: fast_tree_state_nelts(dd) floats.
+long fast_tree_state_nelts(int dd);
+void launch_stage1_fast(Dtype dt, const Stage1Args &a, hipStream_t st);
+void launch_stage2_fast(Dtype dt, const Stage2Args &a, hipStream_t st);
 
 
 // ---- peak finder over (B, ndm_out, M, rowlen) rows -> out_max, out_argmax ----
