@@ -324,6 +324,8 @@ void GpuDedisperser::allocate()
                 tot += cap[r];
             }
         ring_bstride[p] = align_up(tot, 64);
+        PDD_CHECK(ring_bstride[p] < (1L << 31), "GpuDedisperser: lag ring of primary tree " << p
+                  << " exceeds 2^31 elements per beam (tree kernels use 32-bit row offsets)");
         rings[p] = dalloc(size_t(total_beams) * ring_bstride[p] * elt_bytes);
         ring_row_off[p] = up_long(off);
         ring_row_cap[p] = up_long(cap);
@@ -355,6 +357,8 @@ void GpuDedisperser::allocate()
         stage2_lags[it] = up_int(lags);
         // Peak-finder input rows hold per-beam history (and final-lag "future" samples)
         // across chunks: one set of rows per beam of the GPU, not per batch.
+        PDD_CHECK(t.ndm_out * T.M * T.rowlen < (1L << 31), "GpuDedisperser: peak-finder rows of tree " << it
+                  << " exceed 2^31 elements per beam");
         sb[it] = dalloc(size_t(total_beams) * t.ndm_out * T.M * T.rowlen * elt_bytes);
         d_main_emit[it] = up_int(T.main_emit);
         d_stag[it] = up_int(T.stag);

@@ -52,18 +52,30 @@ struct Sub {
 
 template<int DD>
 struct FastDims {
+    static constexpr int D = 1 << DD;
     static constexpr int K1 = (DD + 1) / 2, K2 = DD / 2;
     static constexpr int NRA = 1 << K1, NRB = 1 << K2;
     static constexpr int NW = cmin(8, cmin(NRA, NRB));     // waves per workgroup
     static constexpr int SA = NRB / NW, SB = NRA / NW;     // sub-trees per wave in phase A / B
     static constexpr int T2 = NRB * (NRB - 1) / 2;
     static constexpr int SUMLAG = T2 * (NRA * (NRA - 1) / 2);
-    static constexpr int LDS_FLOATS = SUMLAG + 64 * (1 << DD);
     static constexpr int NSTATE = NRB * Sub<K1>::NST + NRA * Sub<K2>::NST + SUMLAG;
+    static_assert((NRB - 1) * (NRA - 1) < 128, "delay-line shift handles lags below 128");
 
-    // delay of phase-A output row (h, j)
+    // Delay lines: row (h, j) starts at h*S + Q(j) and holds Lm(j) + 64 floats, where
+    // Lm(j) = (NRB-1)*bitrev(j) is the largest delay over h. Phase A writes a tile at
+    // h*S + Q(j) + Lm(j); phase B reads row (h, j) delayed by lag(h, j) at h*S + Q(j) +
+    // Lm(j) - lag(h, j) = Q(j) + h*(S + bitrev(j)).
+    static constexpr int Lm(int j) { return (NRB - 1) * brev_c(j, K1); }
+    static constexpr int Q(int j) { return (j == 0) ? 0 : Q(j - 1) + Lm(j - 1) + 64; }
+    static constexpr int S = Q(NRA);
+    static constexpr int DL_FLOATS = ((NRB * S + 3) / 4) * 4;
+    static constexpr int LDS_BYTES = 4 * DL_FLOATS + 2 * 16 * D;   // + two int4 row-descriptor tables
+    // rank 7: 62 KB of LDS allows 2 workgroups (16 waves) per CU, so aim for 128 VGPRs
+    static constexpr int WAVES_PER_EU = (DD == 7) ? 4 : 2;
+
     __device__ static int lag(int h, int j) { return (NRB - 1 - h) * brev_c(j, K1); }
-    // rows are laid out j-major; row (h, j) holds lag(h, j) + 64 floats; P = sum of earlier lags
+    // residue of row (h, j) in the state buffer: rows j-major, h inner, lag(h, j) floats each
     __device__ static int lagsum(int h, int j)
     {
         int sb = 0;
@@ -71,7 +83,6 @@ struct FastDims {
             sb += brev_c(jj, K1);
         return T2 * sb + brev_c(j, K1) * (h * (NRB - 1) - h * (h - 1) / 2);
     }
-    __device__ static int off(int h, int j) { return lagsum(h, j) + 64 * (j * NRB + h); }
 };
 
 // Stream value at time t - L (L = 0..64): cur at lane t - L, or prev (the previous tile)
@@ -132,36 +143,65 @@ __device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *d
     }
 }
 
-// Phase-B delay-line rows of sub-tree j: load / save the part that carries over chunks.
-template<int DD, bool LOAD>
-__device__ __forceinline__ void move_residue(float *lbuf, float *res, int j, int lane)
-{
+// Per-wave view of the phase-B delay lines of sub-tree j.
+template<int DD>
+struct PhaseB {
     using F = FastDims<DD>;
-    for (int h = 0; h < F::NRB; h++) {
-        const int L = F::lag(h, j), P = F::lagsum(h, j), o = F::off(h, j);
-        for (int q = lane; q < L; q += 64) {
-            if (LOAD)
-                lbuf[o + q] = res[P + q];
-            else
-                res[P + q] = lbuf[o + q];
+    int j, bj, base, step;     // base = Q(j) + lane, step = S + bitrev(j)
+
+    __device__ void init(int jj, int lane)
+    {
+        j = jj;
+        bj = brev_c(j, F::K1);
+        int q = 0;
+        for (int k = 0; k < j; k++)
+            q += F::Lm(k) + 64;
+        base = q + lane;
+        step = F::S + bj;
+    }
+
+    // Row (h, j) at the delayed time of this tile's lane, then shift the line down by 64
+    // (positions [64, 64 + lag) -> [0, lag)); in-order LDS access within the wave keeps the
+    // overlapping copy correct.
+    __device__ __forceinline__ void read(float *lbuf, float (&y)[F::NRB], int lane) const
+    {
+#pragma unroll
+        for (int h = 0; h < F::NRB; h++) {
+            float *p = lbuf + base + h * step;
+            const int L = (F::NRB - 1 - h) * bj;
+            y[h] = p[0];
+            if (lane < L)
+                p[0] = p[64];
+            if (lane + 64 < L)
+                p[64] = p[128];
         }
     }
-}
 
-// Phase B input of sub-tree j: row (h, j) at time t is at delay-line position t; then the
-// line is shifted down by 64 (positions [64, 64 + lag) -> [0, lag)). In-order LDS access
-// within the wave makes the overlapping shift safe (ascending 64-sample blocks).
-template<int DD>
-__device__ __forceinline__ void phase_b_read(float *lbuf, int j, float (&y)[FastDims<DD>::NRB], int lane)
+    template<bool LOAD>
+    __device__ void residue(float *lbuf, float *res, int lane) const
+    {
+        for (int h = 0; h < F::NRB; h++) {
+            float *p = lbuf + base - lane + h * step;
+            float *r = res + F::lagsum(h, j);
+            const int L = F::lag(h, j);
+            for (int q = lane; q < L; q += 64) {
+                if (LOAD)
+                    p[q] = r[q];
+                else
+                    r[q] = p[q];
+            }
+        }
+    }
+};
+
+template<int DD, int K1 = FastDims<DD>::K1>
+__device__ __forceinline__ void phase_a_write(float *lbuf, int h, const float (&x)[1 << K1], int lane)
 {
     using F = FastDims<DD>;
+    float *w = lbuf + h * F::S + lane;
 #pragma unroll
-    for (int h = 0; h < F::NRB; h++) {
-        const int o = F::off(h, j), L = F::lag(h, j);
-        y[h] = lbuf[o + lane];
-        for (int q = lane; q < L; q += 64)
-            lbuf[o + q] = lbuf[o + 64 + q];
-    }
+    for (int j = 0; j < F::NRA; j++)
+        w[F::Q(j) + F::Lm(j)] = x[j];
 }
 
 }  // namespace
@@ -184,77 +224,122 @@ long fast_tree_state_nelts(int dd)
 
 // ---------------------------------------------------------------------------------------
 // Stage 1: (B, 2^amb trees, 2^dd rows, Tp) -> lag ring rows (tree, final row)
+// Stage 2 (final-pass streams only): ring rows (fc, j) at their lag -> peak-finder rows
+//
+// Per-row addressing is precomputed once per launch into LDS descriptors (int4):
+//   ring row (output of stage 1, input of stage 2): x = element offset of the row's position
+//   for chunk time 0 of this chunk (row offset + position), y = number of samples before the
+//   position wraps (capacity - position), z = capacity. Offsets are 32-bit (checked on host).
+//   peak-finder row (stage 2 output): x = element offset of (row, tpad + final lag) or -1.
 
-template<typename Tst, int DD>
-__global__ void __launch_bounds__(FastDims<DD>::NW * 64)
-k_stage1_fast(Stage1Args a)
+template<typename Tst, int DD, bool STAGE2, typename Args>
+__device__ __forceinline__ void fast_tree_body(const Args &a)
 {
     using F = FastDims<DD>;
     using SA_ = Sub<F::K1>;
     using SB_ = Sub<F::K2>;
+    constexpr int D = F::D;
     extern __shared__ float lbuf[];
+    int4 *idesc = reinterpret_cast<int4 *>(lbuf + F::DL_FLOATS);    // stage 2: input rows
+    int4 *odesc = idesc + D;                                          // output rows
+
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ia = blockIdx.x, b = blockIdx.y;
-    constexpr int D = 1 << DD;
+    const int tr = blockIdx.x, b = blockIdx.y;
+    const long gbeam = long(a.beam0) + b;
 
-    float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(ia) * F::NSTATE;
+    // ---- row descriptors
+    for (int i = threadIdx.x; i < D; i += blockDim.x) {
+        if constexpr (!STAGE2) {
+            const long rr = long(tr) * D + i;
+            const long cap = a.ring.row_cap[rr];
+            const long pos = (a.chunk * a.Tp) % cap;
+            odesc[i] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
+        } else {
+            const long rr = (long(i) << a.amb) + tr;
+            const long cap = a.ring.row_cap[rr];
+            const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+            idesc[i] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
+            const int sidx = a.out_stream[i];
+            const long dpf_base = long(bitrev_dev(tr, a.amb)) << (DD - a.pf_rank);
+            odesc[i] = make_int4((sidx < 0) ? -1 : int((dpf_base * a.M + sidx) * a.rowlen + a.tpad + a.fl[sidx]), 0, 0, 0);
+        }
+    }
+
+    // ---- state: histories (registers) and delay-line residue (LDS)
+    float *st = a.state + gbeam * a.state_bstride + long(tr) * F::NSTATE;
     float *st_b = st + F::NRB * SA_::NST;
     float *st_res = st_b + F::NRA * SB_::NST;
 
     float ha[F::SA][SA_::NH], hb[F::SB][SB_::NH];
+    PhaseB<DD> pb[F::SB];
 #pragma unroll
     for (int s = 0; s < F::SA; s++)
         load_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane);
 #pragma unroll
     for (int s = 0; s < F::SB; s++) {
         const int j = wid + s * F::NW;
+        pb[s].init(j, lane);
         load_hist<F::K2>(hb[s], st_b + j * SB_::NST, lane);
-        move_residue<DD, true>(lbuf, st_res, j, lane);
+        pb[s].template residue<true>(lbuf, st_res, lane);
+    }
+    __syncthreads();   // descriptors
+
+    const Tst *src;    // stage 1: this tree's input rows; stage 2: the beam's ring
+    Tst *dst;          // stage 1: the beam's ring; stage 2: the beam's peak-finder rows
+    if constexpr (!STAGE2) {
+        src = reinterpret_cast<const Tst *>(a.in) + long(b) * a.in_bstride + long(tr) * D * a.Tp;
+        dst = reinterpret_cast<Tst *>(a.ring.base) + gbeam * a.ring.bstride;
+    } else {
+        src = reinterpret_cast<const Tst *>(a.ring.base) + gbeam * a.ring.bstride;
+        dst = reinterpret_cast<Tst *>(a.sb) + gbeam * a.sb_bstride;
     }
 
-    const Tst *in = reinterpret_cast<const Tst *>(a.in) + long(b) * a.in_bstride + long(ia) * D * a.Tp;
-    Tst *ring = reinterpret_cast<Tst *>(a.ring.base) + long(a.beam0 + b) * a.ring.bstride;
-
-    // ring write positions of this wave's output rows: (chunk*Tp + t0) mod capacity
-    int opos[F::SB][F::NRB];
+    // phase-A input rows of tile t0 (issued one tile ahead)
+    float xa[F::SA][F::NRA];
+    auto load_a = [&](int t0) {
 #pragma unroll
-    for (int s = 0; s < F::SB; s++)
+        for (int s = 0; s < F::SA; s++) {
+            const int h = wid + s * F::NW;
 #pragma unroll
-        for (int hp = 0; hp < F::NRB; hp++) {
-            const long rr = long(ia) * D + hp * F::NRA + wid + s * F::NW;
-            opos[s][hp] = int((a.chunk * a.Tp) % a.ring.row_cap[rr]);
+            for (int l = 0; l < F::NRA; l++) {
+                if constexpr (!STAGE2) {
+                    xa[s][l] = ld_st(src + long(h * F::NRA + l) * a.Tp + t0 + lane);
+                } else {
+                    const int4 d = idesc[h * F::NRA + l];
+                    const int tl = t0 + lane;
+                    xa[s][l] = ld_st(src + (d.x + tl - ((tl >= d.y) ? d.z : 0)));
+                }
+            }
         }
+    };
+    load_a(0);
 
     for (int t0 = 0; t0 < a.Tp; t0 += 64) {
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
-            const int h = wid + s * F::NW;
-            float x[F::NRA];
-#pragma unroll
-            for (int l = 0; l < F::NRA; l++)
-                x[l] = ld_st(in + long(h * F::NRA + l) * a.Tp + t0 + lane);
-            sub_passes<F::K1>(x, ha[s], lane);
-#pragma unroll
-            for (int j = 0; j < F::NRA; j++)
-                lbuf[F::off(h, j) + F::lag(h, j) + lane] = x[j];
+            sub_passes<F::K1>(xa[s], ha[s], lane);
+            phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
         }
         __syncthreads();
+        if (t0 + 64 < a.Tp)
+            load_a(t0 + 64);
 
 #pragma unroll
         for (int s = 0; s < F::SB; s++) {
-            const int j = wid + s * F::NW;
             float y[F::NRB];
-            phase_b_read<DD>(lbuf, j, y, lane);
+            pb[s].read(lbuf, y, lane);
             sub_passes<F::K2>(y, hb[s], lane);
 #pragma unroll
             for (int hp = 0; hp < F::NRB; hp++) {
-                const long rr = long(ia) * D + hp * F::NRA + j;
-                const int cap = int(a.ring.row_cap[rr]);
-                int p = opos[s][hp];
-                st_st(ring + a.ring.row_off[rr] + p + lane, y[hp]);   // capacities are multiples of 64
-                p += 64;
-                opos[s][hp] = (p >= cap) ? (p - cap) : p;
+                const int4 d = odesc[hp * F::NRA + pb[s].j];
+                if constexpr (!STAGE2) {
+                    // ring capacities and positions are multiples of 64: a tile never wraps
+                    st_st(dst + (d.x + t0 - ((t0 >= d.y) ? d.z : 0) + lane), y[hp]);
+                } else {
+                    if (d.x >= 0)
+                        st_st(dst + (long(d.x) + t0 + lane), y[hp]);
+                }
             }
         }
         __syncthreads();
@@ -265,104 +350,23 @@ k_stage1_fast(Stage1Args a)
         save_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane);
 #pragma unroll
     for (int s = 0; s < F::SB; s++) {
-        const int j = wid + s * F::NW;
-        save_hist<F::K2>(hb[s], st_b + j * SB_::NST, lane);
-        move_residue<DD, false>(lbuf, st_res, j, lane);
+        save_hist<F::K2>(hb[s], st_b + pb[s].j * SB_::NST, lane);
+        pb[s].template residue<false>(lbuf, st_res, lane);
     }
 }
 
-
-// ---------------------------------------------------------------------------------------
-// Stage 2 (final-pass streams only): ring rows (fc, j) at their lag -> peak-finder rows
+template<typename Tst, int DD>
+__global__ void __launch_bounds__(FastDims<DD>::NW * 64) __attribute__((amdgpu_waves_per_eu(FastDims<DD>::WAVES_PER_EU)))
+k_stage1_fast(Stage1Args a)
+{
+    fast_tree_body<Tst, DD, false>(a);
+}
 
 template<typename Tst, int DD>
-__global__ void __launch_bounds__(FastDims<DD>::NW * 64)
+__global__ void __launch_bounds__(FastDims<DD>::NW * 64) __attribute__((amdgpu_waves_per_eu(FastDims<DD>::WAVES_PER_EU)))
 k_stage2_fast(Stage2Args a)
 {
-    using F = FastDims<DD>;
-    using SA_ = Sub<F::K1>;
-    using SB_ = Sub<F::K2>;
-    extern __shared__ float lbuf[];
-    const int lane = threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int jt = blockIdx.x, b = blockIdx.y;
-
-    float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(jt) * F::NSTATE;
-    float *st_b = st + F::NRB * SA_::NST;
-    float *st_res = st_b + F::NRA * SB_::NST;
-
-    float ha[F::SA][SA_::NH], hb[F::SB][SB_::NH];
-#pragma unroll
-    for (int s = 0; s < F::SA; s++)
-        load_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane);
-#pragma unroll
-    for (int s = 0; s < F::SB; s++) {
-        const int j = wid + s * F::NW;
-        load_hist<F::K2>(hb[s], st_b + j * SB_::NST, lane);
-        move_residue<DD, true>(lbuf, st_res, j, lane);
-    }
-
-    const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + long(a.beam0 + b) * a.ring.bstride;
-    Tst *sbb = reinterpret_cast<Tst *>(a.sb) + long(a.beam0 + b) * a.sb_bstride
-               + long(bitrev_dev(jt, a.amb)) * (1 << (DD - a.pf_rank)) * a.M * a.rowlen + a.tpad;
-
-    // ring read positions of this wave's input rows: (chunk*Tp - lag) mod capacity
-    int ipos[F::SA][F::NRA];
-#pragma unroll
-    for (int s = 0; s < F::SA; s++)
-#pragma unroll
-        for (int l = 0; l < F::NRA; l++) {
-            const long rr = (long((wid + s * F::NW) * F::NRA + l) << a.amb) + jt;
-            ipos[s][l] = int(ring_pos(a.chunk * a.Tp - a.row_lag[rr], a.ring.row_cap[rr]));
-        }
-
-    for (int t0 = 0; t0 < a.Tp; t0 += 64) {
-#pragma unroll
-        for (int s = 0; s < F::SA; s++) {
-            const int h = wid + s * F::NW;
-            float x[F::NRA];
-#pragma unroll
-            for (int l = 0; l < F::NRA; l++) {
-                const long rr = (long(h * F::NRA + l) << a.amb) + jt;
-                const int cap = int(a.ring.row_cap[rr]);
-                int p = ipos[s][l] + lane;
-                p = (p >= cap) ? (p - cap) : p;
-                x[l] = ld_st(ring + a.ring.row_off[rr] + p);
-                int pn = ipos[s][l] + 64;
-                ipos[s][l] = (pn >= cap) ? (pn - cap) : pn;
-            }
-            sub_passes<F::K1>(x, ha[s], lane);
-#pragma unroll
-            for (int j = 0; j < F::NRA; j++)
-                lbuf[F::off(h, j) + F::lag(h, j) + lane] = x[j];
-        }
-        __syncthreads();
-
-#pragma unroll
-        for (int s = 0; s < F::SB; s++) {
-            const int j = wid + s * F::NW;
-            float y[F::NRB];
-            phase_b_read<DD>(lbuf, j, y, lane);
-            sub_passes<F::K2>(y, hb[s], lane);
-#pragma unroll
-            for (int hp = 0; hp < F::NRB; hp++) {
-                const int sidx = a.out_stream[hp * F::NRA + j];
-                if (sidx >= 0)
-                    st_st(sbb + long(sidx) * a.rowlen + t0 + a.fl[sidx] + lane, y[hp]);
-            }
-        }
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int s = 0; s < F::SA; s++)
-        save_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane);
-#pragma unroll
-    for (int s = 0; s < F::SB; s++) {
-        const int j = wid + s * F::NW;
-        save_hist<F::K2>(hb[s], st_b + j * SB_::NST, lane);
-        move_residue<DD, false>(lbuf, st_res, j, lane);
-    }
+    fast_tree_body<Tst, DD, true>(a);
 }
 
 
@@ -373,7 +377,7 @@ static void launch_stage1_fast_t(const Stage1Args &a, hipStream_t st)
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
         hipLaunchKernelGGL((k_stage1_fast<Tst, DDV>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
-                           sizeof(float) * F::LDS_FLOATS, st, a);                            \
+                           F::LDS_BYTES, st, a);                            \
         break;                                                                               \
     }
     switch (a.dd) {
@@ -390,7 +394,7 @@ static void launch_stage2_fast_t(const Stage2Args &a, hipStream_t st)
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
         hipLaunchKernelGGL((k_stage2_fast<Tst, DDV>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
-                           sizeof(float) * F::LDS_FLOATS, st, a);                            \
+                           F::LDS_BYTES, st, a);                            \
         break;                                                                               \
     }
     switch (a.dd) {
