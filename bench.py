@@ -1,4 +1,4 @@
-"""Benchmark: real-time beams of the MI355X dedispersion chain on BASELINE config 2.
+"""Benchmark: real-time beams of the MI355X dedispersion chain at CHORD-like scale.
 
 Workload (one "step" = one chunk of one beam batch through the whole chain: tree gridding,
 lagged downsampling, stage-1 trees, lag ring, stage-2 trees + subbands, peak finding):
@@ -11,8 +11,10 @@ trees (1/2/4/8 ms), chunks of 2048 samples, float32 storage and arithmetic.
 in the dedisperser's input slot (HBM) before the timed region. Per-kernel times come from
 HIP events recorded on the dedisperser's compute stream (a separate, shorter pass), and
 the roofline object reports the dominant kernel's algorithmic bytes / average duration
-against MI355X's 8 TB/s HBM peak. The CPU baseline runs the oracle (a single-threaded C++
-restatement of the reference's CPU dedisperser) on one beam of the same shapes.
+against MI355X's 8 TB/s HBM peak; its "traffic" (HBM bytes per launch) comes from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed in profiles/ for this workload, when
+present. The CPU baseline runs the oracle (a single-threaded C++ restatement of the
+reference's CPU dedisperser) on one beam of the same shapes.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -31,6 +33,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, 'tests'))
 
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, 'profiles', 'chain_chord16k_fp32_rocprof.json')
 K_DM = 4.148808e6   # ms MHz^2 / (pc cm^-3)
 
 
@@ -114,6 +117,16 @@ def cpu_baseline(cfg_dict, nchunks=8, seed=0):
     return dict(value=round(1 * chunk_s / dt, 4), unit='beams', cores=1, kind='port',
                 sample=f'1 beam x {nchunks} chunks of the same shapes (16384 ch, rank 14, 4 trees, T=2048, fp32), '
                        f'{dt:.3f} s per beam-chunk on one core of {platform.processor() or platform.machine()}')
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch (read + write) of 'kernel' from the committed rocprofv3 summary."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            rec = json.load(f)['kernels'][kernel]
+        return int(rec['hbm_read_bytes'] + rec['hbm_write_bytes'])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -221,13 +234,14 @@ def main():
         'dtype': 'f32',
         'data': 'synthetic: N(0,1) noise + 8 dispersed boxcar pulses per beam (DM U(0,3000), width logU(1,16) ms, '
                 'spectral index U(-2,2)), one resident chunk reused every step',
-        'config': {'workload': 'BASELINE config 2 (plain FP32 GPU path): 16 beams x 16384 ch 300-1500 MHz, 1 ms, '
+        'config': {'workload': 'chord16k plain FP32 chain: 16 beams x 16384 ch 300-1500 MHz, 1 ms, '
                                'tree rank 14, 4 primary trees (1/2/4/8 ms), chunk 2048, no subbands, max_width 16/8/4/2',
                    'beams_per_batch': B, 'nfreq': F, 'nt_chunk': T, 'toplevel_tree_rank': cfg.toplevel_tree_rank,
                    'num_primary_trees': plan.num_primary_trees, 'chain_alg_GBps': round(chain_gbps, 1),
                    'chain_alg_bytes_per_step': int(chain_alg_bytes(plan)), 'per_kernel': per_kernel},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': per_kernel[dom]['GBps'], 'peak': HBM_PEAK_GBPS,
-                     'unit': 'GB/s', 'frac': round(per_kernel[dom]['GBps'] / HBM_PEAK_GBPS, 4), 'traffic': None},
+                     'unit': 'GB/s', 'frac': round(per_kernel[dom]['GBps'] / HBM_PEAK_GBPS, 4),
+                     'traffic': measured_traffic(dom)},
     }
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(CFG2)

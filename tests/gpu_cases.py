@@ -3,7 +3,7 @@
 Each case is a dedispersion config in the reference's YAML grammar (as a dict), small enough
 for the CPU oracle to check in seconds, chosen to cover: one/several primary trees, early
 triggers, subband searches with level-0, staggered and full-band multiplets, several
-frequency zones, beam batching (several batches), and the BASELINE config-1 shape.
+frequency zones, beam batching (several batches), and the 1-beam 1024-channel rank-10 shape.
 """
 
 BASE = dict(time_sample_ms=1.0, dtype='float32', future_write_max_samples=0)
@@ -19,7 +19,7 @@ CASES = {
     'toy': dict(BASE, zone_nfreq=[640], zone_freq_edges=[400, 800], toplevel_tree_rank=8, time_samples_per_chunk=256,
                 beams_per_gpu=4, beams_per_batch=2, num_active_batches=2, frequency_subband_counts=[2, 2, 1],
                 primary_trees=[pt(0, 16, 16, 16), pt(1, 8, 16, 16), pt(1, 4, 16, 16)]),
-    # One primary tree, no subbands (BASELINE config-1 structure, shorter chunk).
+    # One primary tree, no subbands (the cfg1 structure with a shorter chunk).
     'one_tree': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                      time_samples_per_chunk=256, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                      frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
@@ -32,14 +32,14 @@ CASES = {
     'subbands': dict(BASE, zone_nfreq=[4096], zone_freq_edges=[300, 1500], toplevel_tree_rank=12,
                      time_samples_per_chunk=512, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
                      frequency_subband_counts=[4, 5, 3, 1], primary_trees=[pt(0, 32, 64, 64), pt(1, 16, 64, 64)]),
-    # 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
+    # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                  frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
 }
 
-# BASELINE config 2 (bench workload): 16 beams x 16384 channels 300-1500 MHz, rank 14, four
-# primary trees (1/2/4/8 ms), chunks of 2048 samples, float32. Unspecified by BASELINE and
+# chord16k (bench workload): 16 beams x 16384 channels 300-1500 MHz, rank 14, four
+# primary trees (1/2/4/8 ms), chunks of 2048 samples, float32. Not fixed by the workload and
 # chosen here: no subband search ([1]), max_width 16/8/4/2 (16 ms at every downsampling),
 # weight downsampling 64x64.
 CFG2 = dict(BASE, zone_nfreq=[16384], zone_freq_edges=[300, 1500], toplevel_tree_rank=14, time_samples_per_chunk=2048,
