@@ -96,7 +96,7 @@ def inject_pulses(x, cfg, rng, npulses=8):
                 x[b].index_put_((ff, tt), torch.from_numpy(amp.astype(np.float32)).to(x.device), accumulate=True)
 
 
-def cpu_baseline(cfg_dict, nchunks=8, seed=0):
+def cpu_baseline(cfg_dict, nchunks=24, seed=0):
     """Oracle (single-threaded C++ restatement of the reference CPU dedisperser,
     sophistication 1) on one beam of the bench shapes; returns real-time beams."""
     from oracle import oracle as orc

@@ -73,6 +73,14 @@ def lib():
         if not os.path.exists(LIBPATH):
             raise RuntimeError(f'pirate_amd: {LIBPATH} is not built; run "python -m pirate_amd.build" '
                                '(or __graft_entry__.build()) first')
+        # torch wheels bundle their own HIP runtime with the same soname (libamdhip64.so.7)
+        # as /opt/rocm's. Loading torch first makes this library bind to the runtime torch
+        # uses, so the process holds one HIP runtime; loaded the other way round, torch
+        # later finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

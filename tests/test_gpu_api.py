@@ -1,0 +1,108 @@
+"""GpuDedisperser call-order semantics on the GPU: cursors, error latching, stop(),
+several consumers, and back-pressure of release_input on unreleased outputs.
+
+Reference: include/pirate/Dedisperser.hpp:170-219 (acquire/release contract, stop),
+src_lib/GpuDedisperser.cpp:742-984 (cursor checks, consumer cursors, back-pressure
+at :1080-1081), pirate_frb/GpuDedisperser.py:85-89.
+"""
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from gpu_cases import CASES
+
+pytestmark = pytest.mark.gpu
+
+
+def make(name='toy', **kw):
+    import pirate_amd as pa
+    cfg = pa.DedispersionConfig.from_dict(CASES[name])
+    plan = pa.DedispersionPlan(cfg)
+    dd = pa.GpuDedisperser(plan, **kw)
+    dd.allocate()
+    for it in range(plan.ntrees):
+        dd.fill_all_weights(it, np.ones(dd.wt_shape(it), np.float32))
+    return dd
+
+
+def push(dd, seq):
+    with dd.get_input(seq) as arr:
+        arr.zero_()
+
+
+def test_input_out_of_order_latches_first_error():
+    dd = make()
+    with pytest.raises(RuntimeError, match='expected seq_id=0'):
+        dd._acquire_input(1, 0)
+    # every later call reports the first error
+    with pytest.raises(RuntimeError, match='expected seq_id=0'):
+        dd._acquire_input(0, 0)
+    with pytest.raises(RuntimeError, match='expected seq_id=0'):
+        dd._acquire_output(0, 0, 0)
+
+
+def test_stop_latches_reason():
+    dd = make()
+    push(dd, 0)
+    dd.stop('operator halt')
+    with pytest.raises(RuntimeError, match='operator halt'):
+        dd._acquire_input(1, 0)
+    with pytest.raises(RuntimeError, match='operator halt'):
+        dd._acquire_output(0, 0, 0)
+
+
+def test_release_output_out_of_order():
+    dd = make()
+    push(dd, 0)
+    out = dd._acquire_output(0, 0, 0, sync=True)
+    assert out.ichunk_zero_based == 0
+    with pytest.raises(RuntimeError, match='expected seq_id=0'):
+        dd._release_output(0, 1, 0)
+
+
+def test_negative_weights_rejected():
+    dd = make()
+    w = np.ones(dd.wt_shape(0), np.float32)
+    w.flat[3] = -1.0
+    with pytest.raises(RuntimeError, match='finite and >= 0'):
+        dd.fill_all_weights(0, w)
+
+
+def test_two_consumers_and_back_pressure():
+    import torch
+    dd = make('toy', num_consumers=2, nbatches_out=1)
+    push(dd, 0)
+    dd._acquire_input(1, 0)
+    res = {'done': False, 'err': None}
+
+    def release():
+        try:
+            dd._release_input_and_launch_dd_kernels(1, 0)
+            res['done'] = True
+        except Exception as e:       # reported below
+            res['err'] = e
+
+    # Output slot 0 (nbatches_out = 1) is held until both consumers release seq 0.
+    th = threading.Thread(target=release, daemon=True)
+    th.start()
+    try:
+        time.sleep(0.5)
+        assert not res['done'], 'release_input_and_launch did not wait for unreleased outputs'
+        outs = [dd._acquire_output(c, 0, 0, sync=True) for c in range(2)]
+        assert torch.equal(outs[0].out_max[0].cpu(), outs[1].out_max[0].cpu())
+        dd._release_output(0, 0, 0)
+        time.sleep(0.2)
+        assert not res['done'], 'release_input_and_launch ran before every consumer released'
+        dd._release_output(1, 0, 0)
+        th.join(timeout=30)
+        assert res['err'] is None and res['done']
+    finally:
+        if th.is_alive():
+            dd.stop('test failed: waking the blocked release')   # stop() wakes every waiter
+            th.join(timeout=10)
+    for c in range(2):
+        dd._acquire_output(c, 1, 0, sync=True)
+        dd._release_output(c, 1, 0)
