@@ -93,7 +93,7 @@ def inject_pulses(x, cfg, rng, npulses=8):
             for w in range(width):
                 tt = torch.from_numpy((t0 + delay + w) % T).to(x.device)
                 ff = torch.arange(F, device=x.device)
-                x[b].index_put_((ff, tt), torch.from_numpy(amp.astype(np.float32)).to(x.device), accumulate=True)
+                x[b].index_put_((ff, tt), torch.from_numpy(amp.astype(np.float32)).to(x.device, x.dtype), accumulate=True)
 
 
 def cpu_baseline(cfg_dict, nchunks=24, seed=0):
@@ -129,6 +129,68 @@ def measured_traffic(kernel):
         return None
 
 
+def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
+    """Runs the chain on one resident chunk; returns (plan, seconds per step, per-kernel dict, GiB)."""
+    cfg = pa.DedispersionConfig.from_dict(cfg_dict)
+    plan = pa.DedispersionPlan(cfg)
+    dd = pa.GpuDedisperser(plan)
+    dd.allocate()
+    gib = dd.gpu_bytes() / 2**30
+    print(f'[bench] {plan.dtype}: allocated {gib:.2f} GiB of device memory', file=sys.stderr, flush=True)
+
+    from oracle import oracle as orc   # weights helper only (bare-kernel weights)
+    for it in range(plan.ntrees):
+        w = orc.bare_weights(*dd.wt_shape(it)[1:])
+        dd.fill_all_weights(it, np.broadcast_to(w, dd.wt_shape(it)))
+
+    rng = np.random.default_rng(2)
+    seq = 0
+    with dd.get_input(seq) as arr:
+        gen = torch.Generator(device=arr.device)
+        gen.manual_seed(2)
+        arr.normal_(generator=gen)
+        inject_pulses(arr, cfg, rng)
+    with dd.get_output(seq):
+        pass
+    seq += 1
+
+    def step():
+        nonlocal seq
+        # The input slot keeps the resident chunk (stale-slot reuse): no host or HBM copy.
+        s = torch.cuda.current_stream().cuda_stream
+        dd._acquire_input(seq, s)
+        dd._release_input_and_launch_dd_kernels(seq, s)
+        dd._acquire_output(0, seq, s)
+        dd._release_output(0, seq, s)
+        seq += 1
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(f'[bench] {plan.dtype}: {steps} steps, {dt * 1e3:.3f} ms/step', file=sys.stderr, flush=True)
+
+    # Per-kernel times (HIP events on the compute stream), separate pass.
+    dd.set_kernel_timing(True)
+    for _ in range(timing_steps):
+        step()
+    torch.cuda.synchronize()
+    kt = dd.kernel_timing()
+    dd.set_kernel_timing(False)
+    per_kernel = {}
+    for name, (n, ms) in kt.items():
+        avg_ms = ms / n
+        nbytes = kernel_alg_bytes(plan, dd, name)
+        per_kernel[name] = dict(avg_ms=round(avg_ms, 4), alg_bytes=int(nbytes), GBps=round(nbytes / (avg_ms * 1e-3) / 1e9, 1))
+    del dd
+    torch.cuda.synchronize()
+    return plan, dt, per_kernel, gib
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -136,6 +198,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--timing-steps', type=int, default=4)
+    ap.add_argument('--no-fp16', action='store_true', help='skip the float16-storage measurement')
     ap.add_argument('--calib-copy', action='store_true',
                     help='run one 1 GiB device-to-device copy first (reference dispatch for HBM counters)')
     args = ap.parse_args()
@@ -157,61 +220,9 @@ def main():
         del src, dst
         print('[bench] calibration copy: 1 GiB read + 1 GiB write', file=sys.stderr, flush=True)
 
-    cfg = pa.DedispersionConfig.from_dict(CFG2)
-    plan = pa.DedispersionPlan(cfg)
-    dd = pa.GpuDedisperser(plan)
-    dd.allocate()
-    print(f'[bench] allocated {dd.gpu_bytes() / 2**30:.2f} GiB of device memory', file=sys.stderr, flush=True)
-
-    from oracle import oracle as orc   # weights helper only (bare-kernel weights)
-    for it in range(plan.ntrees):
-        w = orc.bare_weights(*dd.wt_shape(it)[1:])
-        dd.fill_all_weights(it, np.broadcast_to(w, dd.wt_shape(it)))
-
-    rng = np.random.default_rng(2)
+    plan, dt, per_kernel, gib = measure(pa, torch, CFG2, args.steps, args.warmup, args.timing_steps)
     B, F, T = plan.beams_per_batch, plan.nfreq, plan.nt_in
-    seq = 0
-    with dd.get_input(seq) as arr:
-        gen = torch.Generator(device=arr.device)
-        gen.manual_seed(2)
-        arr.normal_(generator=gen)
-        inject_pulses(arr, cfg, rng)
-    with dd.get_output(seq):
-        pass
-    seq += 1
-
-    def step():
-        nonlocal seq
-        # The input slot keeps the resident chunk (stale-slot reuse): no host or HBM copy.
-        s = torch.cuda.current_stream().cuda_stream
-        dd._acquire_input(seq, s)
-        dd._release_input_and_launch_dd_kernels(seq, s)
-        dd._acquire_output(0, seq, s)
-        dd._release_output(0, seq, s)
-        seq += 1
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    print(f'[bench] {args.steps} steps, {dt * 1e3:.3f} ms/step', file=sys.stderr, flush=True)
-
-    # Per-kernel times (HIP events on the compute stream), separate pass.
-    dd.set_kernel_timing(True)
-    for _ in range(args.timing_steps):
-        step()
-    torch.cuda.synchronize()
-    kt = dd.kernel_timing()
-    dd.set_kernel_timing(False)
-    per_kernel = {}
-    for name, (n, ms) in kt.items():
-        avg_ms = ms / n
-        nbytes = kernel_alg_bytes(plan, dd, name)
-        per_kernel[name] = dict(avg_ms=round(avg_ms, 4), alg_bytes=int(nbytes), GBps=round(nbytes / (avg_ms * 1e-3) / 1e9, 1))
+    cfg = plan.config
     dom = max(per_kernel, key=lambda k: per_kernel[k]['avg_ms'])
     for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]['avg_ms']):
         print(f'[bench] kernel {k:12s} {v["avg_ms"]:8.3f} ms  {v["GBps"]:8.1f} GB/s (alg bytes {v["alg_bytes"] / 1e6:.1f} MB)',
@@ -243,6 +254,15 @@ def main():
                      'unit': 'GB/s', 'frac': round(per_kernel[dom]['GBps'] / HBM_PEAK_GBPS, 4),
                      'traffic': measured_traffic(dom)},
     }
+    if not args.no_fp16:
+        # Same workload with float16 storage (float32 arithmetic): the reduced-precision tier.
+        p16, dt16, pk16, gib16 = measure(pa, torch, dict(CFG2, dtype='float16'), max(args.steps // 2, 1),
+                                         args.warmup, args.timing_steps)
+        dom16 = max(pk16, key=lambda k: pk16[k]['avg_ms'])
+        result['fp16'] = {'value': round(B * chunk_s / dt16, 2), 'unit': 'beams', 'ms_per_step': round(dt16 * 1e3, 4),
+                          'ratio_vs_fp32': round(dt / dt16, 3), 'dtype': 'f16 storage, f32 arithmetic',
+                          'gpu_GiB': round(gib16, 2), 'dominant_kernel': dom16,
+                          'dominant_alg_GBps': pk16[dom16]['GBps'], 'per_kernel': pk16}
     if not args.no_cpu_baseline:
         result['cpu_baseline'] = cpu_baseline(CFG2)
     print(json.dumps(result), flush=True)

@@ -44,6 +44,23 @@ __device__ __forceinline__ float4 ld4_st(const __hip_bfloat16 *p)
                        __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
 }
 
+// Four consecutive storage elements from float (round-to-nearest-even for 16-bit types).
+__device__ __forceinline__ void st4_st(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+
+__device__ __forceinline__ void st4_st(__half *p, float4 v)
+{
+    const unsigned a = __half_as_ushort(__float2half(v.x)), b = __half_as_ushort(__float2half(v.y));
+    const unsigned c = __half_as_ushort(__float2half(v.z)), d = __half_as_ushort(__float2half(v.w));
+    *reinterpret_cast<uint2 *>(p) = make_uint2(a | (b << 16), c | (d << 16));
+}
+
+__device__ __forceinline__ void st4_st(__hip_bfloat16 *p, float4 v)
+{
+    const unsigned a = __bfloat16_as_ushort(__float2bfloat16(v.x)), b = __bfloat16_as_ushort(__float2bfloat16(v.y));
+    const unsigned c = __bfloat16_as_ushort(__float2bfloat16(v.z)), d = __bfloat16_as_ushort(__float2bfloat16(v.w));
+    *reinterpret_cast<uint2 *>(p) = make_uint2(a | (b << 16), c | (d << 16));
+}
+
 __device__ __forceinline__ int bitrev_dev(int i, int nbits)
 {
     return (nbits > 0) ? int(__brev(unsigned(i)) >> (32 - nbits)) : 0;

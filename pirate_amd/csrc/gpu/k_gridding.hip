@@ -18,21 +18,12 @@ static constexpr int kGridNPB = 8;
 
 template<typename Tst>
 struct Vec4 {
-    __device__ static void load(const Tst *p, float *x) { for (int i = 0; i < 4; i++) x[i] = ld_st(p + i); }
-    __device__ static void store(Tst *p, const float *x) { for (int i = 0; i < 4; i++) st_st(p + i, x[i]); }
-};
-
-template<>
-struct Vec4<float> {
-    __device__ static void load(const float *p, float *x)
+    __device__ static void load(const Tst *p, float *x)
     {
-        float4 v = *reinterpret_cast<const float4 *>(p);
+        const float4 v = ld4_st(p);
         x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
     }
-    __device__ static void store(float *p, const float *x)
-    {
-        *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
-    }
+    __device__ static void store(Tst *p, const float *x) { st4_st(p, make_float4(x[0], x[1], x[2], x[3])); }
 };
 
 template<typename Tst>

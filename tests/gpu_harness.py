@@ -61,3 +61,64 @@ def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print):
                         bad.append(rec)
                         log(f'MISMATCH {rec}')
     return bad, dd, plan
+
+
+def quantize(x, dtype):
+    """Rounds float32 x to the storage dtype (round-to-nearest-even) and returns it as float64."""
+    import torch
+    tdt = {'float16': torch.float16, 'bfloat16': torch.bfloat16, 'float32': torch.float32}[dtype]
+    return torch.from_numpy(x).to(tdt).to(torch.float64).numpy()
+
+
+def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print):
+    """Runs the GPU chain with 16-bit storage and the FP64 oracle on the same (quantized) inputs
+    with bare-kernel weights (outputs in SNR units for unit-variance noise). Returns per-tree
+    error statistics: max and rms of |gpu - fp64| / max(|fp64|, 1) over out_max, and the same
+    for the FP64 value of the GPU's argmax token (how far the chosen candidate is from the max)."""
+    import torch
+    import pirate_amd as pa
+
+    d16 = dict(cfg_dict, dtype=dtype)
+    cfg = pa.DedispersionConfig.from_dict(d16)
+    plan = pa.DedispersionPlan(cfg)
+    plan32 = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(dict(cfg_dict, dtype='float32')))
+    for t16, t32 in zip(plan.trees, plan32.trees):
+        assert vars(t16) == vars(t32), 'storage dtype changed the tree plan'
+    dd = pa.GpuDedisperser(plan)
+    dd.allocate()
+    ref = orc.Dedisperser(oracle_config(dict(cfg_dict, dtype='float32')), 1, dbl=True)
+    W = [orc.bare_weights(*dd.wt_shape(it)[1:]) for it in range(plan.ntrees)]
+    for it in range(plan.ntrees):
+        dd.fill_all_weights(it, np.broadcast_to(W[it], dd.wt_shape(it)).copy())
+        ref.set_weights(it, W[it])
+    tdt = {'float16': torch.float16, 'bfloat16': torch.bfloat16}[dtype]
+
+    rng = np.random.default_rng(seed)
+    B, F, T = plan.beams_per_batch, plan.nfreq, plan.nt_in
+    errs = {it: [] for it in range(plan.ntrees)}
+    tok_errs = {it: [] for it in range(plan.ntrees)}
+    for ichunk in range(nchunks):
+        for ib in range(dd.nbatches):
+            seq = ichunk * dd.nbatches + ib
+            x = rng.standard_normal((B, F, T)).astype(np.float32)
+            xq = quantize(x, dtype)
+            with dd.get_input(seq) as arr:
+                arr.copy_(torch.from_numpy(xq).to(tdt).to(arr.device))
+            ref.dedisperse(xq, ichunk, ib)
+            with dd.get_output(seq) as out:
+                for it in range(plan.ntrees):
+                    gm = out.out_max[it].to(torch.float64).cpu().numpy()
+                    ga = out.out_argmax[it].cpu().numpy().view(np.uint32)
+                    rm, _ = ref.output(it)
+                    scale = np.maximum(np.abs(rm), 1.0)
+                    errs[it].append((np.abs(gm - rm) / scale).ravel())
+                    ev = ref.eval_tokens(it, ga)
+                    tok_errs[it].append((np.abs(ev - rm) / scale).ravel())
+    stats = {}
+    for it in range(plan.ntrees):
+        e = np.concatenate(errs[it])
+        te = np.concatenate(tok_errs[it])
+        stats[it] = dict(max=float(e.max()), rms=float(np.sqrt(np.mean(e**2))), tok_max=float(te.max()), n=int(e.size))
+        log(f'{dtype} tree {it}: max rel err {stats[it]["max"]:.2e}, rms {stats[it]["rms"]:.2e}, '
+            f'argmax candidate max rel err {stats[it]["tok_max"]:.2e} over {e.size} (dm, t) outputs')
+    return stats

@@ -1,0 +1,31 @@
+"""Reduced-precision storage (float16 / bfloat16 storage, float32 arithmetic) against the
+FP64 oracle: every (DM, time) output of every tree, on unit-variance noise with bare-kernel
+weights, so out_max is in SNR units. The inputs are rounded to the storage type first and
+the FP64 oracle sees the same rounded values, so the errors below are those of the chain.
+
+Tolerances (relative to max(|SNR|, 1)): float16 max 1e-2 (measured ~1e-3); bfloat16 max
+5e-2 and rms 1e-2 (8-bit significand: each of the ~4 storage roundings on the path is up to
+2^-9 relative). The GPU's argmax candidate, re-evaluated in FP64, must be within the same
+tolerance of the FP64 maximum.
+"""
+
+import pytest
+
+from gpu_cases import CASES
+from gpu_harness import run_precision
+
+pytestmark = pytest.mark.gpu
+
+TOL = {'float16': dict(max=1e-2, rms=2e-3), 'bfloat16': dict(max=5e-2, rms=1e-2)}
+
+
+@pytest.mark.parametrize('dtype', ['float16', 'bfloat16'])
+@pytest.mark.parametrize('name', ['cfg1', 'toy', 'zones4'])
+def test_reduced_precision_vs_fp64_oracle(name, dtype):
+    nchunks = 2 if name == 'cfg1' else 4
+    stats = run_precision(CASES[name], dtype, nchunks, seed=7)
+    tol = TOL[dtype]
+    for it, s in stats.items():
+        assert s['max'] <= tol['max'], (it, s)
+        assert s['rms'] <= tol['rms'], (it, s)
+        assert s['tok_max'] <= tol['max'], (it, s)
