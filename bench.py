@@ -1,7 +1,7 @@
 """Benchmark: real-time beams of the MI355X dedispersion chain at CHORD-like scale.
 
-Workload (one "step" = one chunk of one beam batch through the whole chain: tree gridding,
-lagged downsampling, stage-1 trees, lag ring, stage-2 trees + subbands, peak finding):
+Workload (one "step" = one chunk of one beam batch through the whole chain: tree gridding
+fused with lagged downsampling, stage-1 trees, lag ring, stage-2 trees + subbands, peak finding):
 16 beams x 16384 channels (300-1500 MHz), 1 ms samples, toplevel tree rank 14, 4 primary
 trees (1/2/4/8 ms), chunks of 2048 samples, float32 storage and arithmetic.
 
@@ -43,10 +43,8 @@ def kernel_alg_bytes(plan, dd, name):
     B, F, T = plan.beams_per_batch, plan.nfreq, plan.nt_in
     R = plan.config.toplevel_tree_rank
     npri = plan.num_primary_trees
-    if name == 'gridding':
-        return B * (F + 2**R) * T * S
-    if name == 'lds':
-        return B * (2**R * T + sum(2**(R - 1) * (T >> p) for p in range(1, npri))) * S
+    if name == 'grid_lds':
+        return B * (F * T + 2**R * T + sum(2**(R - 1) * (T >> p) for p in range(1, npri))) * S
     if name.startswith('stage1_p'):
         p = int(name[len('stage1_p'):])
         rp = R - (1 if p else 0)

@@ -137,10 +137,22 @@ int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *host_w
 /* GpuDedisperser::stop(e), GpuDedisperser.cpp:520-577. */
 int pdd_stop(pdd_dedisperser *dd, const char *reason);
 
+/* ---- input stage ---- */
+
+/* GpuDequantizationKernel::launch (include/pirate/GpuDequantizationKernel.hpp:56-65; semantics of
+ * ReferenceDequantizationKernel::apply, src_lib/GpuDequantizationKernel.cu:326-380).
+ * data: (nbeams, nfreq, ntime) int4, two per byte, low nibble = even sample, -8 = missing (-> 0);
+ * scales_offsets: (nbeams, nfreq, ntime/256, 2) float16 {scale, offset}; out: (nbeams, nfreq, ntime)
+ * of out_dtype ("float32", "float16" or "bfloat16"); all device pointers, ntime % 256 == 0.
+ * out = scale * value + offset computed in float32 (exactly the reference CPU kernel's value),
+ * rounded to 16 bits for 16-bit outputs. Asynchronous on 'stream'. */
+int pdd_dequantize(const char *out_dtype, int64_t nbeams, int64_t nfreq, int64_t ntime, const void *data,
+                   const void *scales_offsets, void *out, void *stream);
+
 /* Kernel timing (reference counterpart: the per-kernel KernelTimer loops and ResourceTracker
  * rates of GpuDedisperser::time, src_lib/GpuDedisperser.cpp:1573-1690). When enabled, HIP
  * events bracket every kernel launch on the compute stream; the report holds one line
- * "name launches total_ms" per kernel (gridding, lds, stage1_p<p>, stage2_t<i>, pf_t<i>). */
+ * "name launches total_ms" per kernel (grid_lds, stage1_p<p>, stage2_t<i>, pf_t<i>). */
 int pdd_kernel_timing(pdd_dedisperser *dd, int enable);
 int pdd_kernel_timing_report(pdd_dedisperser *dd, char *buf, size_t buflen, size_t *needed);
 

@@ -318,3 +318,13 @@ extern "C" int orc_dd_eval_tokens(void *hv, long itree, const uint32_t *tokens, 
         else { auto *d = get<RefDedisperser<float>>(h); d->pf.at(itree)->eval_tokens((float *) out, tokens, d->wt.at(itree).data()); }
     });
 }
+
+extern "C" int orc_dequantize(const uint8_t *data, const uint16_t *scoff, long nbeams, long nfreq, long ntime, float *out)
+{
+    if (ntime % 256) {
+        g_err = "orc_dequantize: ntime must be a multiple of 256";
+        return 1;
+    }
+    ref_dequantize(data, scoff, nbeams, nfreq, ntime, out);
+    return 0;
+}

@@ -915,3 +915,61 @@ struct RefDedisperser {
 }  // namespace oracle
 
 #endif
+
+
+// -------------------------------------------------------------------------------------------
+// Dequantization of packed int4 input (reference: ReferenceDequantizationKernel::apply,
+// src_lib/GpuDequantizationKernel.cu:326-380; format in include/pirate/GpuDequantizationKernel.hpp:17-43).
+// data: (nbeams, nfreq, ntime) int4, two per byte, low nibble = even sample; value -8 is the
+// missing-sample sentinel and maps to 0. scoff: (nbeams, nfreq, ntime/256, 2) IEEE binary16
+// bit patterns {scale, offset}, converted to float before any arithmetic.
+// out = scale * value + offset (float32).
+
+inline float half_bits_to_float(uint16_t h)
+{
+    // IEEE 754 binary16 -> binary32 (exact): sign, 5-bit exponent (bias 15), 10-bit fraction.
+    const uint32_t sign = uint32_t(h >> 15) << 31;
+    const int e = (h >> 10) & 0x1f;
+    uint32_t frac = h & 0x3ff;
+    uint32_t bits;
+    if (e == 0x1f)
+        bits = sign | 0x7f800000u | (frac << 13);                 // inf / nan
+    else if (e != 0)
+        bits = sign | (uint32_t(e - 15 + 127) << 23) | (frac << 13);
+    else if (frac == 0)
+        bits = sign;                                               // signed zero
+    else {
+        int ee = -14;                                              // subnormal: normalize
+        while (!(frac & 0x400)) {
+            frac <<= 1;
+            ee--;
+        }
+        bits = sign | (uint32_t(ee + 127) << 23) | ((frac & 0x3ff) << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+inline void ref_dequantize(const uint8_t *data, const uint16_t *scoff, long nbeams, long nfreq, long ntime, float *out)
+{
+    const long nchunks = ntime / 256;
+    for (long b = 0; b < nbeams; b++)
+        for (long f = 0; f < nfreq; f++)
+            for (long c = 0; c < nchunks; c++) {
+                const long pair = (b * nfreq + f) * nchunks + c;
+                const float scale = half_bits_to_float(scoff[2 * pair]);
+                const float offset = half_bits_to_float(scoff[2 * pair + 1]);
+                for (long k = 0; k < 256; k++) {
+                    const long i = pair * 256 + k;
+                    const int nib = (data[i >> 1] >> (4 * (i & 1))) & 0xf;
+                    const int v = (nib >= 8) ? nib - 16 : nib;
+                    float x = 0.0f;
+                    if (v != -8) {
+                        const float p = scale * float(v);
+                        x = p + offset;
+                    }
+                    out[i] = x;
+                }
+            }
+}

@@ -357,3 +357,17 @@ def sparse_random_weights(rng, B, ndm_wt, nt_wt, P, N):
     r = rng.uniform(0.0, 1.0, size=(B, ndm_wt, nt_wt, P, N)).astype(np.float32)
     x = rng.uniform(0.0, 1.0, size=(B, ndm_wt, nt_wt, P, N)).astype(np.float32)
     return np.where(r < p0, x, np.float32(0.0)).astype(np.float32) * base[None, None, None]
+
+
+def dequantize(data_u8, scoff_f16, nbeams, nfreq, ntime):
+    """Reference dequantization (ReferenceDequantizationKernel::apply,
+    src_lib/GpuDequantizationKernel.cu:326-380). data_u8: uint8 (nbeams, nfreq, ntime/2), two int4
+    per byte (low nibble = even sample); scoff_f16: float16 (nbeams, nfreq, ntime/256, 2).
+    Returns float32 (nbeams, nfreq, ntime)."""
+    d = np.ascontiguousarray(data_u8, dtype=np.uint8)
+    s = np.ascontiguousarray(scoff_f16, dtype=np.float16).view(np.uint16)
+    assert d.size * 2 == nbeams * nfreq * ntime and s.size == nbeams * nfreq * (ntime // 256) * 2
+    out = np.zeros((nbeams, nfreq, ntime), dtype=np.float32)
+    _check(lib().orc_dequantize(_ptr(d), _ptr(s), ctypes.c_long(nbeams), ctypes.c_long(nfreq),
+                                ctypes.c_long(ntime), _ptr(out)))
+    return out

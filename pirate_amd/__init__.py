@@ -7,6 +7,7 @@ over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library wit
 
 from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
 from .dedisperser import GpuDedisperser, GpuDedisperserOutputs
+from .kernels import GpuDequantizationKernel
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
-           'GpuDedisperser', 'GpuDedisperserOutputs']
+           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel']

@@ -57,6 +57,7 @@ _SIGS = {
     'pdd_release_input_and_launch': (ctypes.c_int, [vp, i64, vp]),
     'pdd_acquire_output': (ctypes.c_int, [vp, i64, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),
     'pdd_release_output': (ctypes.c_int, [vp, i64, i64, vp]),
+    'pdd_dequantize': (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, vp, vp, vp, vp]),
     'pdd_fill_all_weights': (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_float), i64]),
     'pdd_stop': (ctypes.c_int, [vp, ctypes.c_char_p]),
     'pdd_kernel_timing': (ctypes.c_int, [vp, ctypes.c_int]),

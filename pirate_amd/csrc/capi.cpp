@@ -5,6 +5,7 @@
 #include "host/config.hpp"
 #include "host/plan.hpp"
 #include "gpu/dedisperser.hpp"
+#include "gpu/kernels.hpp"
 
 #include <cstring>
 #include <memory>
@@ -291,6 +292,23 @@ int pdd_debug_copy(pdd_dedisperser *dd, const char *name, void *host, size_t max
         size_t n = dd->d->debug_copy(name, host, maxbytes);
         if (nbytes)
             *nbytes = n;
+    });
+}
+
+int pdd_dequantize(const char *out_dtype, int64_t nbeams, int64_t nfreq, int64_t ntime, const void *data,
+                   const void *scales_offsets, void *out, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(out_dtype && data && scales_offsets && out, "pdd_dequantize: null argument");
+        PDD_CHECK(nbeams > 0 && nfreq > 0 && ntime > 0, "pdd_dequantize: shape must be positive");
+        PDD_CHECK(ntime % 256 == 0, "pdd_dequantize: ntime must be a multiple of 256 (got " << ntime << ")");
+        const Dtype dt = dtype_from_str(out_dtype);
+        PDD_CHECK((reinterpret_cast<uintptr_t>(out) % 16) == 0 && (reinterpret_cast<uintptr_t>(data) % 4) == 0
+                      && (reinterpret_cast<uintptr_t>(scales_offsets) % 4) == 0,
+                  "pdd_dequantize: misaligned pointer (out: 16 bytes, data and scales_offsets: 4 bytes)");
+        launch_dequant(dt, data, scales_offsets, out, nbeams * nfreq * ntime, (hipStream_t) stream);
+        hipError_t e = hipGetLastError();
+        PDD_CHECK(e == hipSuccess, "pdd_dequantize: launch failed: " << hipGetErrorString(e));
     });
 }
 

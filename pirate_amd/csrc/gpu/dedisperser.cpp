@@ -400,11 +400,11 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
     const int beam0 = int(ibatch * B);
 
     {
-        GriddingArgs g;
+        GridLdsArgs g;
         g.in = input;
-        g.out = grid_buf.ptr;
         g.in_bstride = nfreq * nt_in;
-        g.out_bstride = N * nt_in;
+        g.grid = grid_buf.ptr;
+        g.grid_bstride = N * nt_in;
         g.B = int(B);
         g.nchan = int(N);
         g.nfreq = int(nfreq);
@@ -413,31 +413,19 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         g.row_cnt = g.row_f0 + N;
         g.row_woff = g.row_f0 + 2 * N;
         g.w = (const float *) grid_w.ptr;
-        hipEvent_t ev;
-        timed_begin("gridding", st, ev);
-        launch_gridding(dtype, g, st);
-        timed_end("gridding", st, ev);
-    }
-    if (npri > 1) {
-        LdsArgs l;
-        l.in = grid_buf.ptr;
-        l.in_bstride = N * nt_in;
-        for (long p = 1; p < npri; p++) {
-            l.out[p] = lds_out[p].ptr;
-            l.out_bstride[p] = (N / 2) * (nt_in >> p);
+        for (long p = 0; p < kMaxPrimaryTrees; p++) {
+            g.out[p] = (p > 0 && p < npri) ? lds_out[p].ptr : nullptr;
+            g.out_bstride[p] = (p > 0 && p < npri) ? (N / 2) * (nt_in >> p) : 0;
         }
-        l.state = (float *) lds_state.ptr;
-        l.state_bstride = lds_state_bstride;
-        l.B = int(B);
-        l.R = int(R);
-        l.npri = int(npri);
-        l.ntime = int(nt_in);
-        l.s = int(plan->stage1_dd_rank[1] + 1);
-        l.beam0 = beam0;
+        g.state = (float *) lds_state.ptr;
+        g.state_bstride = lds_state_bstride;
+        g.npri = int(npri);
+        g.s = (npri > 1) ? int(plan->stage1_dd_rank[1] + 1) : 1;
+        g.beam0 = beam0;
         hipEvent_t ev;
-        timed_begin("lds", st, ev);
-        launch_lds(dtype, l, st);
-        timed_end("lds", st, ev);
+        timed_begin("grid_lds", st, ev);
+        launch_grid_lds(dtype, g, st);
+        timed_end("grid_lds", st, ev);
     }
     for (long p = 0; p < npri; p++) {
         Stage1Args s;

@@ -69,7 +69,7 @@ struct FastDims {
     static constexpr int Lm(int j) { return (NRB - 1) * brev_c(j, K1); }
     static constexpr int Q(int j) { return (j == 0) ? 0 : Q(j - 1) + Lm(j - 1) + 64; }
     static constexpr int S = Q(NRA);
-    static constexpr int DL_FLOATS = ((NRB * S + 3) / 4) * 4;
+    static constexpr int DL_FLOATS = ((NRB * S + 128 + 3) / 4) * 4;   // + padding for the shift reads
     static constexpr int LDS_BYTES = 4 * DL_FLOATS + 2 * 16 * D;   // + two int4 row-descriptor tables
     // rank 7: 62 KB of LDS allows 2 workgroups (16 waves) per CU, so aim for 128 VGPRs
     static constexpr int WAVES_PER_EU = (DD == 7) ? 4 : 2;
@@ -162,7 +162,9 @@ struct PhaseB {
 
     // Row (h, j) at the delayed time of this tile's lane, then shift the line down by 64
     // (positions [64, 64 + lag) -> [0, lag)); in-order LDS access within the wave keeps the
-    // overlapping copy correct.
+    // overlapping copy correct. Only positions below lag must be right: [lag, lag + 64) is
+    // the next tile's phase-A window, so lanes at or above lag may copy garbage (reads past
+    // the row stay inside the padded buffer), which keeps the copy branch-free.
     __device__ __forceinline__ void read(float *lbuf, float (&y)[F::NRB], int lane) const
     {
 #pragma unroll
@@ -170,9 +172,8 @@ struct PhaseB {
             float *p = lbuf + base + h * step;
             const int L = (F::NRB - 1 - h) * bj;
             y[h] = p[0];
-            if (lane < L)
-                p[0] = p[64];
-            if (lane + 64 < L)
+            p[0] = p[64];
+            if (L > 64)          // wave-uniform
                 p[64] = p[128];
         }
     }
@@ -313,17 +314,14 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
             }
         }
     };
-    load_a(0);
-
     for (int t0 = 0; t0 < a.Tp; t0 += 64) {
+        load_a(t0);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             sub_passes<F::K1>(xa[s], ha[s], lane);
             phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
         }
         __syncthreads();
-        if (t0 + 64 < a.Tp)
-            load_a(t0 + 64);
 
 #pragma unroll
         for (int s = 0; s < F::SB; s++) {

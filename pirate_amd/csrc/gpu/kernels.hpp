@@ -9,35 +9,30 @@
 
 namespace pirate_amd {
 
-// ---- tree gridding: out[b,n,t] = sum_f w[n,f] in[b,f,t] (CSR weights, ascending f) ----
-struct GriddingArgs {
-    const void *in;
-    void *out;
-    long in_bstride, out_bstride;
-    int B, nchan, nfreq, ntime;
-    const int *row_f0;      // (nchan) first input channel of tree channel n
-    const int *row_cnt;     // (nchan) number of input channels
-    const int *row_woff;    // (nchan) offset into w
-    const float *w;         // overlap weights
-};
-void launch_gridding(Dtype dt, const GriddingArgs &a, hipStream_t s);
-
-
-// ---- lagged downsampling, primary trees p = 1..npri-1 ----
+// ---- tree gridding fused with lagged downsampling ----
+// grid[b,n,t] = sum_f w[n,f] in[b,f,t] (CSR weights, ascending f), n < nchan = 2^R, and for
+// primary trees p = 1..npri-1: out[p] (B, 2^(R-1), ntime >> p).
 static constexpr int kMaxPrimaryTrees = 8;
 
-struct LdsArgs {
-    const void *in;                      // (B, 2^R, T) level-0 tree-gridded
+struct GridLdsArgs {
+    const void *in;                      // (B, nfreq, ntime) input slot
     long in_bstride;
-    void *out[kMaxPrimaryTrees];         // out[p], p >= 1: (B, 2^(R-1), T >> p)
+    void *grid;                          // (B, nchan, ntime) gridded rows (primary tree 0 input)
+    long grid_bstride;
+    int B, nchan, nfreq, ntime;
+    const int *row_f0;                   // (nchan) first input channel of tree channel n
+    const int *row_cnt;                  // (nchan) number of input channels
+    const int *row_woff;                 // (nchan) offset into w
+    const float *w;                      // overlap weights
+    void *out[kMaxPrimaryTrees];         // out[p], p >= 1
     long out_bstride[kMaxPrimaryTrees];
     float *state;                        // per global beam, see lds_state_nelts()
     long state_bstride;
-    int B, R, npri, ntime, s;            // s = (stage-1 dd_rank of p>=1) + 1
+    int npri, s;                         // s = (stage-1 dd_rank of p>=1) + 1
     int beam0;                           // global index of the batch's first beam
 };
 long lds_state_nelts(int R, int npri, int s);   // per beam
-void launch_lds(Dtype dt, const LdsArgs &a, hipStream_t st);
+void launch_grid_lds(Dtype dt, const GridLdsArgs &a, hipStream_t st);
 
 
 // ---- stage-1 tree: (B, 2^amb, 2^dd, Tp) -> lag ring rows (a, dm_brev) ----
@@ -93,6 +88,11 @@ void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st);
 long fast_tree_state_nelts(int dd);
 void launch_stage1_fast(Dtype dt, const Stage1Args &a, hipStream_t st);
 void launch_stage2_fast(Dtype dt, const Stage2Args &a, hipStream_t st);
+
+
+// ---- int4 dequantization: out = scale * value + offset per 256-sample (beam, freq) block ----
+// data: nsamples int4 (two per byte, low nibble first); scoff: float16 {scale, offset} pairs.
+void launch_dequant(Dtype dt, const void *data, const void *scoff, void *out, long nsamples, hipStream_t st);
 
 
 // ---- peak finder over (B, ndm_out, M, rowlen) rows -> out_max, out_argmax ----

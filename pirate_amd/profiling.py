@@ -5,7 +5,7 @@ Usage (after profiling bench.py, see --help):
     python -m pirate_amd.profiling --stats DIR [--fetch DIR --write DIR] --md OUT.md --json OUT.json
 
 DIR is a rocprofv3 output directory written with ``--output-format csv -o run``. Kernel
-dispatches of this library are named by their place in the chain (gridding, lds,
+dispatches of this library are named by their place in the chain (grid_lds,
 stage1_p<p>, stage2_t<i>, pf_t<i>), in launch order: one chain per step, so the i-th
 dispatch of the library within a step has a fixed name. HBM counters are scaled by what the
 same counter reads on a reference copy of known size (the 1 GiB copy that bench.py makes
@@ -37,8 +37,7 @@ def _short(name):
 
 
 def chain_names(npri, ntrees):
-    names = ['gridding'] + (['lds'] if npri > 1 else [])
-    names += ['stage1_p%d' % p for p in range(npri)]
+    names = ['grid_lds'] + ['stage1_p%d' % p for p in range(npri)]
     for i in range(ntrees):
         names += ['stage2_t%d' % i, 'pf_t%d' % i]
     return names
