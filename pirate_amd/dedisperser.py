@@ -47,6 +47,21 @@ def _stream_ptr(stream):
     return stream.cuda_stream
 
 
+def bare_profile_variances(P):
+    """Zero-lag autocorrelation of each peak-finding profile for unit-variance white input
+    (what the reference's PfVarianceConvolver returns for a single unit sample,
+    src_lib/PfVariance.cpp): profile 0 is one sample; profiles 3l+1, 3l+2, 3l+3 are the kernels
+    [1,1], [1/2,1,1/2], [1/2,1,1,1/2] on the level-l grid, each tap a boxcar of 2^l samples."""
+    v = np.zeros(P, dtype=np.float64)
+    for p in range(P):
+        if p == 0:
+            v[p] = 1.0
+        else:
+            l, q = (p - 1) // 3, (p - 1) % 3 + 1
+            v[p] = 2.0**l * {1: 2.0, 2: 1.5, 3: 2.5}[q]
+    return v
+
+
 class GpuDedisperserOutputs:
     """Outputs of one beam batch: out_max[itree], out_argmax[itree] (tensor views valid until
     the matching release). Reference: pirate_frb/core/GpuDedisperserOutputs.py."""
@@ -178,6 +193,36 @@ class GpuDedisperser:
         if w.shape != self.wt_shape(itree):
             raise ValueError(f'fill_all_weights: expected shape {self.wt_shape(itree)}, got {w.shape}')
         L.check(L.lib().pdd_fill_all_weights(self._h, itree, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size))
+
+    def fill_host_weights(self, itree, variances=None, randomize=False, rng=None):
+        """Builds and uploads tree itree's weights the way the reference's
+        PeakFindingKernelParams::fill_host_weights does (src_lib/PeakFindingKernel.cu:588-691):
+        base[d, n, p] = 1/sqrt(variances[n, d, p]) for variances of shape (N, ndm_wt, nprofiles),
+        or, with variances=None, the bare-kernel weights 1/sqrt(zero-lag autocorrelation of
+        profile p) that give SNR units for unit-variance input. randomize=True multiplies each
+        weight by a sparse random factor (per (beam, d, t) an occupancy p0 ~ U(0.01, 1.1); each
+        (p, n) weight is 0 with probability ~(1 - p0), else U(0, 1)), drawn from 'rng' (numpy
+        Generator). Returns the (nbatches_wt, B, ndm_wt, nt_wt, P, N) float32 array uploaded."""
+        t = self.trees[itree]
+        nb, B, D, T, P, N = self.wt_shape(itree)
+        if variances is None:
+            base = np.broadcast_to((1.0 / np.sqrt(bare_profile_variances(P))).astype(np.float32)[None, None, :], (D, N, P))
+        else:
+            v = np.asarray(variances, dtype=np.float64)
+            if v.shape != (N, D, P):
+                raise ValueError(f'fill_host_weights: variances must have shape (N, ndm_wt, P) = {(N, D, P)}, got {v.shape}')
+            if not np.all(v > 0):
+                raise ValueError('fill_host_weights: variances must be positive')
+            base = (np.float32(1.0) / np.sqrt(v.astype(np.float32))).transpose(1, 0, 2)      # (D, N, P)
+        w = np.broadcast_to(base.transpose(0, 2, 1)[None, None, :, None, :, :], (nb, B, D, T, P, N)).astype(np.float32)
+        if randomize:
+            rng = rng if rng is not None else np.random.default_rng()
+            p0 = rng.uniform(0.01, 1.1, size=(nb, B, D, T, 1, 1))
+            r = rng.uniform(0.0, 1.0, size=w.shape)
+            x = rng.uniform(0.0, 1.0, size=w.shape)
+            w = (np.where(r < p0, x, 0.0).astype(np.float32) * w).astype(np.float32)
+        self.fill_all_weights(itree, w)
+        return w
 
     def stop(self, reason=''):
         L.check(L.lib().pdd_stop(self._h, reason.encode()))

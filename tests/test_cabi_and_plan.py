@@ -186,3 +186,9 @@ def test_gpu_limits_reported_cleanly():
     plan = pa.DedispersionPlan(cfg)
     with pytest.raises(_lib.PddError, match='not supported by this build'):
         pa.GpuDedisperser(plan)
+
+
+def test_bare_profile_variances_match_oracle(orc):
+    from pirate_amd.dedisperser import bare_profile_variances
+    for P in (1, 4, 7, 10, 13, 16):
+        assert np.array_equal(bare_profile_variances(P), orc.bare_profile_variances(P))

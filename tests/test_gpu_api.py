@@ -106,3 +106,18 @@ def test_two_consumers_and_back_pressure():
     for c in range(2):
         dd._acquire_output(c, 1, 0, sync=True)
         dd._release_output(c, 1, 0)
+
+
+def test_fill_host_weights_bare_and_variances():
+    from oracle import oracle as orc
+    dd = make('toy')
+    for it in range(dd.ntrees):
+        nb, B, D, T, P, N = dd.wt_shape(it)
+        w = dd.fill_host_weights(it)
+        assert np.array_equal(w, np.broadcast_to(orc.bare_weights(B, D, T, P, N), w.shape))
+        var = np.random.default_rng(it).uniform(0.5, 2.0, size=(N, D, P))
+        w = dd.fill_host_weights(it, var)
+        want = (np.float32(1) / np.sqrt(var.astype(np.float32))).transpose(1, 2, 0)      # (D, P, N)
+        assert np.array_equal(w[0, 0, :, 0], want)
+        w = dd.fill_host_weights(it, var, randomize=True, rng=np.random.default_rng(1))
+        assert np.all(w >= 0) and np.all(w <= np.broadcast_to(want[None, None, :, None], w.shape))
