@@ -1,5 +1,6 @@
 #include "dedisperser.hpp"
 #include "../host/common.hpp"
+#include "../host/pf_variance.hpp"
 
 #include <algorithm>
 #include <cmath>

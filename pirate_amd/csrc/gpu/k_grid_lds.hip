@@ -9,15 +9,17 @@
 //     out_p[j][t'] = c*(ds_p[2j][t-1] + ds_p[2j+1][t]),  t' = t + L_j     (one-sample lag on
 //                    the even row, then the "large" lag L_j = 2^(s-1) - 1 - (j mod 2^(s-1)))
 //
-// Mapping: one workgroup of 4 waves per tree-channel pair (2j, 2j+1); wave w takes time
-// tiles w, w+4, ... of 64 lanes x G samples (G = max(4, 2^(npri-1))), so a workgroup streams
-// whole rows. Lane l owns G consecutive samples of both rows: it grids them from the input
-// channels (16-byte loads), stores the gridded rows (the input of primary tree 0), and keeps
-// them in registers for the downsampling levels, so the gridded chunk is never read back
-// from HBM. The one-sample lag crosses lanes with a shuffle; lane 0 of each wave takes the
-// previous tile's last value through LDS (the previous chunk's, from per-beam state, for the
-// first tile). The large lag is a write offset: outputs past the chunk end go to per-beam
-// state and are copied to the next chunk's first L_j samples.
+// Mapping: one wave per (tree-channel pair (2j, 2j+1), time tile of 64 lanes x G samples),
+// G = max(4, 2^(npri-1)); a workgroup holds 4 consecutive tiles of one pair. Waves never
+// wait on each other (no LDS, no barriers). Lane l owns G consecutive samples of both rows:
+// it grids them from the input channels (16-byte loads), stores the gridded rows (the input
+// of primary tree 0), and keeps them in registers for the downsampling levels, so the
+// gridded chunk is never read back from HBM. The one-sample lag crosses lanes with a
+// shuffle; lane 0 recomputes the previous group of the even row itself (for the first tile:
+// the previous chunk's last value, from per-beam state). The large lag is a write offset:
+// outputs past the chunk end go to per-beam state and are copied to the next chunk's first
+// L_j samples. The state is double-buffered by chunk parity, so the first tile's reads and
+// the last tile's writes of one launch never touch the same memory.
 // With 16-bit storage the gridded values are rounded to the storage type before the
 // downsampling levels use them (as if stored and re-read); the levels themselves stay in
 // float32 and round only at their output stores.
@@ -26,6 +28,7 @@
 // neighbouring channel pairs, which share boundary input channels, run on the same XCD and
 // share its L2.
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -41,7 +44,7 @@ long lds_state_nelts(int R, int npri, int s)
         return 0;
     long nrows = 1L << (R - 1);
     long lcap = 1L << (s - 1);
-    return (npri - 1) * nrows * (1 + lcap);
+    return 2 * (npri - 1) * nrows * (1 + lcap);     // two parities
 }
 
 template<typename Tst>
@@ -112,16 +115,20 @@ k_grid_lds(GridLdsArgs a)
 {
     constexpr int G = ((1 << (NPRI - 1)) > 4) ? (1 << (NPRI - 1)) : 4;
     constexpr int NL = (NPRI > 1) ? NPRI : 2;          // array size for per-level values
-    __shared__ float lastv[kGlWaves][NL];              // last even-row value of each wave's tile, per level
-    __shared__ float carry[NL];                        // value before the current round's first tile
 
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int ngroups = a.ntime / G;
+    const int ntiles = (ngroups + 63) / 64;
+    const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;          // workgroups per channel pair
     const int nblk = gridDim.x;
     const int xb = blockIdx.x;
-    const int j = (nblk % 8 == 0) ? ((xb % 8) * (nblk / 8) + xb / 8) : xb;    // channel pair
+    const int blk = (nblk % 8 == 0) ? ((xb % 8) * (nblk / 8) + xb / 8) : xb;
+    const int j = blk / tpb;                                       // channel pair
+    const int tile = (blk - j * tpb) * kGlWaves + (threadIdx.x >> 6);
     const int b = blockIdx.y;
     const int nrows = a.nchan >> 1;
+    if (tile >= ntiles)
+        return;
 
     const int fe = a.row_f0[2 * j], ce = a.row_cnt[2 * j];
     const int fo = a.row_f0[2 * j + 1], co = a.row_cnt[2 * j + 1];
@@ -131,127 +138,130 @@ k_grid_lds(GridLdsArgs a)
     Tst *gout_e = reinterpret_cast<Tst *>(a.grid) + long(b) * a.grid_bstride + long(2 * j) * a.ntime;
     Tst *gout_o = gout_e + a.ntime;
 
-    // lagged-downsampling state (NPRI > 1): outputs that overflowed the previous chunk, and
-    // the last even-row value of the previous chunk at every level
-    const int lmask = (1 << (a.s - 1)) - 1;
-    const int Lj = lmask - (j & lmask);
-    const int lcap = lmask + 1;
-    float *st = a.state + long(a.beam0 + b) * a.state_bstride;
-    float *st_small = st;                                     // [(p-1)*nrows + j]
-    float *st_large = st + long(NPRI - 1) * nrows;            // [((p-1)*nrows + j)*lcap + k]
-    Tst *outp[NL];
+    const int g0 = tile * 64;
+    const int g = g0 + lane;
+    const int nact = min(64, ngroups - g0);
+    const bool active = (lane < nact);
+    const int t0 = g * G;
+
+    float xe[G], xo[G];
+    if (active) {
+        grid_pair<Tst, G>(in + t0, a.ntime, fe, ce, we, fo, co, wo, xe, xo);
 #pragma unroll
-    for (int p = 1; p < NPRI; p++) {
-        const int Tp = a.ntime >> p;
-        outp[p] = reinterpret_cast<Tst *>(a.out[p]) + long(b) * a.out_bstride[p] + long(j) * Tp;
-        if (w == p % kGlWaves) {
-            const float *sl = st_large + (long(p - 1) * nrows + j) * lcap;
-            for (int k = lane; k < Lj; k += 64)
-                st_st(outp[p] + k, sl[k]);
+        for (int q = 0; q < G; q += 4) {
+            st4_st(gout_e + t0 + q, make_float4(xe[q], xe[q + 1], xe[q + 2], xe[q + 3]));
+            st4_st(gout_o + t0 + q, make_float4(xo[q], xo[q + 1], xo[q + 2], xo[q + 3]));
         }
-        if (threadIdx.x == 0)
-            carry[p] = st_small[long(p - 1) * nrows + j];
-    }
-    if constexpr (NPRI > 1)
-        __syncthreads();     // state reads above precede the overflow writes of the last tile
-
-    const int ngroups = a.ntime / G;
-    const int ntiles = (ngroups + 63) / 64;
-    for (int k0 = 0; k0 < ntiles; k0 += kGlWaves) {
-        const int tile = k0 + w;
-        const bool tile_on = tile < ntiles;
-        const int g0 = tile * 64;
-        const int g = g0 + lane;
-        const int nact = tile_on ? min(64, ngroups - g0) : 0;
-        const bool active = (lane < nact);
-        const int t0 = g * G;
-
-        float xe[G], xo[G];
-        if (active) {
-            grid_pair<Tst, G>(in + t0, a.ntime, fe, ce, we, fo, co, wo, xe, xo);
+    } else {
 #pragma unroll
-            for (int q = 0; q < G; q += 4) {
-                st4_st(gout_e + t0 + q, make_float4(xe[q], xe[q + 1], xe[q + 2], xe[q + 3]));
-                st4_st(gout_o + t0 + q, make_float4(xo[q], xo[q + 1], xo[q + 2], xo[q + 3]));
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < G; i++)
-                xe[i] = xo[i] = 0.0f;
-        }
-
-        if constexpr (NPRI > 1) {
-#pragma unroll
-            for (int i = 0; i < G; i++) {
-                xe[i] = quant<Tst>(xe[i]);
-                xo[i] = quant<Tst>(xo[i]);
-            }
-            float xo0[NL];     // lane 0: odd-row value of its first output, per level (needs the carry)
-#pragma unroll
-            for (int p = 1; p < NPRI; p++) {
-                const int n = G >> p;
-#pragma unroll
-                for (int k = 0; k < (G >> 1); k++) {
-                    if (k < n) {
-                        xe[k] = kRsqrt2 * (xe[2 * k] + xe[2 * k + 1]);
-                        xo[k] = kRsqrt2 * (xo[2 * k] + xo[2 * k + 1]);
-                    }
-                }
-                if (tile_on && lane == nact - 1)
-                    lastv[w][p] = xe[n - 1];
-                const float prev = __shfl_up(xe[n - 1], 1);
-                xo0[p] = xo[0];
-
-                const int Tp = a.ntime >> p;
-                const int tp = g * n;
-                float *sl = st_large + (long(p - 1) * nrows + j) * lcap;
-                const bool all_in = (g0 + nact) * n + Lj <= Tp;    // wave-uniform
-                if (active) {
-#pragma unroll
-                    for (int k = 0; k < (G >> 1); k++) {
-                        if (k < n && (k > 0 || lane > 0)) {
-                            const float se = (k == 0) ? prev : xe[k - 1];
-                            const float o = kRsqrt2 * (se + xo[k]);
-                            const int pos = tp + k + Lj;
-                            if (all_in || pos < Tp)
-                                st_st(outp[p] + pos, o);
-                            else
-                                sl[pos - Tp] = o;
-                        }
-                    }
-                }
-            }
-            __syncthreads();     // lastv of every wave of this round
-            if (tile_on && lane == 0) {
-#pragma unroll
-                for (int p = 1; p < NPRI; p++) {
-                    const int n = G >> p;
-                    const int Tp = a.ntime >> p;
-                    const float prev = (w == 0) ? carry[p] : lastv[w - 1][p];
-                    const float o = kRsqrt2 * (prev + xo0[p]);
-                    const int pos = g0 * n + Lj;
-                    if (pos < Tp)
-                        st_st(outp[p] + pos, o);
-                    else
-                        st_large[(long(p - 1) * nrows + j) * lcap + pos - Tp] = o;
-                }
-            }
-            __syncthreads();     // every lane 0 has read carry / lastv
-            if (threadIdx.x == 0) {
-                const int wl = min(kGlWaves, ntiles - k0) - 1;     // last wave with a tile this round
-#pragma unroll
-                for (int p = 1; p < NPRI; p++)
-                    carry[p] = lastv[wl][p];
-            }
-            __syncthreads();
-        }
+        for (int i = 0; i < G; i++)
+            xe[i] = xo[i] = 0.0f;
     }
 
     if constexpr (NPRI > 1) {
-        if (threadIdx.x == 0) {
+        // Per-beam state, double-buffered by chunk parity so that no wave reads what another
+        // wave of the same launch writes: [parity][small (p-1, j) | large ((p-1, j), lcap)].
+        const int lmask = (1 << (a.s - 1)) - 1;
+        const int Lj = lmask - (j & lmask);
+        const int lcap = lmask + 1;
+        const long half = long(NPRI - 1) * nrows * (1 + lcap);
+        float *st = a.state + long(a.beam0 + b) * a.state_bstride;
+        const float *rd = st + (a.chunk & 1) * half;            // written by the previous chunk
+        float *wr = st + ((a.chunk + 1) & 1) * half;            // read by the next chunk
+        const float *rd_small = rd, *rd_large = rd + long(NPRI - 1) * nrows;
+        float *wr_small = wr, *wr_large = wr + long(NPRI - 1) * nrows;
+
+        Tst *outp[NL];
 #pragma unroll
-            for (int p = 1; p < NPRI; p++)
-                st_small[long(p - 1) * nrows + j] = carry[p];
+        for (int p = 1; p < NPRI; p++)
+            outp[p] = reinterpret_cast<Tst *>(a.out[p]) + long(b) * a.out_bstride[p] + long(j) * (a.ntime >> p);
+
+        // Lane 0: the even row's level-p value just before this tile (previous chunk's last
+        // value for the first tile, else recomputed from the previous group of samples).
+        float prevs[NL];
+        if (lane == 0) {
+            if (tile == 0) {
+#pragma unroll
+                for (int p = 1; p < NPRI; p++)
+                    prevs[p] = rd_small[long(p - 1) * nrows + j];
+            } else {
+                float xp[G], xq[G];
+                grid_pair<Tst, G>(in + t0 - G, a.ntime, fe, ce, we, fe, 0, we, xp, xq);
+#pragma unroll
+                for (int i = 0; i < G; i++)
+                    xp[i] = quant<Tst>(xp[i]);
+#pragma unroll
+                for (int p = 1; p < NPRI; p++) {
+                    const int n = G >> p;
+#pragma unroll
+                    for (int k = 0; k < (G >> 1); k++)
+                        if (k < n)
+                            xp[k] = kRsqrt2 * (xp[2 * k] + xp[2 * k + 1]);
+                    prevs[p] = xp[n - 1];
+                }
+            }
+        }
+        // The first tile also lays down the outputs that overflowed the previous chunk.
+        if (tile == 0) {
+#pragma unroll
+            for (int p = 1; p < NPRI; p++) {
+                const float *sl = rd_large + (long(p - 1) * nrows + j) * lcap;
+                for (int k = lane; k < Lj; k += 64)
+                    st_st(outp[p] + k, sl[k]);
+            }
+        }
+
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            xe[i] = quant<Tst>(xe[i]);
+            xo[i] = quant<Tst>(xo[i]);
+        }
+        const bool last_tile = (tile == ntiles - 1);
+        // Per-wave LDS transpose buffer: a lane's n consecutive outputs are written to LDS and
+        // read back so that each global store instruction writes 64 consecutive samples.
+        __shared__ float tbuf[kGlWaves][64 * (G / 2)];
+        float *tb = tbuf[threadIdx.x >> 6];
+#pragma unroll
+        for (int p = 1; p < NPRI; p++) {
+            const int n = G >> p;
+#pragma unroll
+            for (int k = 0; k < (G >> 1); k++) {
+                if (k < n) {
+                    xe[k] = kRsqrt2 * (xe[2 * k] + xe[2 * k + 1]);
+                    xo[k] = kRsqrt2 * (xo[2 * k] + xo[2 * k + 1]);
+                }
+            }
+            float prev = __shfl_up(xe[n - 1], 1);
+            if (lane == 0)
+                prev = prevs[p];
+            if (last_tile && lane == nact - 1)
+                wr_small[long(p - 1) * nrows + j] = xe[n - 1];
+
+#pragma unroll
+            for (int k = 0; k < (G >> 1); k++)
+                if (k < n)
+                    tb[lane * n + k] = kRsqrt2 * (((k == 0) ? prev : xe[k - 1]) + xo[k]);
+            __builtin_amdgcn_wave_barrier();
+
+            const int Tp = a.ntime >> p;
+            const int base = g0 * n + Lj;                // position of this tile's first output
+            const int nout = nact * n;
+            float *sl = wr_large + (long(p - 1) * nrows + j) * lcap;
+#pragma unroll
+            for (int q = 0; q < (G >> 1); q++) {
+                if (q < n) {
+                    const int i = 64 * q + lane;
+                    const float o = tb[i];
+                    const int pos = base + i;
+                    if (i < nout) {
+                        if (pos < Tp)
+                            st_st(outp[p] + pos, o);
+                        else
+                            sl[pos - Tp] = o;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();     // the buffer is reused by the next level
         }
     }
 }
@@ -259,7 +269,10 @@ k_grid_lds(GridLdsArgs a)
 template<typename Tst>
 static void launch_grid_lds_t(const GridLdsArgs &a, hipStream_t st)
 {
-    dim3 grid(a.nchan >> 1, a.B);      // one workgroup per tree-channel pair
+    const int G = std::max(4, 1 << (a.npri - 1));
+    const int ntiles = (a.ntime / G + 63) / 64;
+    const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;
+    dim3 grid((a.nchan >> 1) * tpb, a.B);      // workgroups of 4 independent (pair, tile) waves
     dim3 block(kGlWaves * 64);
     switch (a.npri) {
         case 1: hipLaunchKernelGGL((k_grid_lds<Tst, 1>), grid, block, 0, st, a); break;

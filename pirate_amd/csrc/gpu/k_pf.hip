@@ -67,8 +67,32 @@ template<typename S>
 struct PfLane {
     float w[S::WTOT];          // per level l: w[woff(l) + i] = B_l(e - i*dt(l)), e = newest end
     float bv[S::NLEV];         // per level: best weighted value in the current bin
-    unsigned int bc[S::NLEV];  // per level: its t | (p << 8)
+    unsigned int bc[S::NLEV];  // per level: its code (see code())
     float wt[S::P];            // weights of the current bin, per profile
+
+    // Candidate codes within a level. Compact codes isamp*4 + q (q = 0 for p = 0, else
+    // p - 3l) stay within the hardware's inline-constant range (<= 64) when Dout <= 16, so the
+    // per-candidate selects need no literal moves; token() expands them once per bin.
+    static constexpr bool COMPACT = (S::DOUT <= 16);
+
+    template<int L>
+    static constexpr unsigned int code(int isamp, int q)
+    {
+        return COMPACT ? unsigned(4 * isamp + q)
+                       : (unsigned(isamp * S::dt(L)) | (unsigned(q ? 3 * L + q : 0) << 8));
+    }
+
+    template<int L>
+    __device__ __forceinline__ unsigned int token(unsigned int c) const
+    {
+        if constexpr (COMPACT) {
+            const unsigned int q = c & 3u, isamp = c >> 2;
+            const unsigned int p = q ? (3u * L + q) : 0u;
+            return (isamp * unsigned(S::dt(L))) | (p << 8);
+        } else {
+            return c;
+        }
+    }
 
     template<int L>
     __device__ __forceinline__ void push(float v)
@@ -97,20 +121,20 @@ struct PfLane {
     __device__ __forceinline__ void candidates()
     {
         constexpr int o = S::woff(L), s = S::sgrid(L);
-        constexpr unsigned int t = unsigned(POS + 1 - S::dt(L));
-        constexpr bool first = (t == 0);
+        constexpr int isamp = (POS + 1) / S::dt(L) - 1;
+        constexpr bool first = (isamp == 0);
         const float x3 = w[o], x2 = w[o + s], x1 = w[o + 2 * s], x0 = w[o + 3 * s];
         if constexpr (L == 0)
-            upd<0, first>(wt[0] * x3, t);
+            upd<0, first>(wt[0] * x3, code<0>(isamp, 0));
         if constexpr (S::P > 1) {
             constexpr int p = 3 * L + 1;
             const float h3 = 0.5f * x3;
             const float y1 = x2 + x3;
             const float y2 = (0.5f * x1 + x2) + h3;
             const float y3 = ((0.5f * x0 + x1) + x2) + h3;
-            upd<L, first && (L > 0)>(wt[p] * y1, t | (unsigned(p) << 8));
-            upd<L, false>(wt[p + 1] * y2, t | (unsigned(p + 1) << 8));
-            upd<L, false>(wt[p + 2] * y3, t | (unsigned(p + 2) << 8));
+            upd<L, first && (L > 0)>(wt[p] * y1, code<L>(isamp, 1));
+            upd<L, false>(wt[p + 1] * y2, code<L>(isamp, 2));
+            upd<L, false>(wt[p + 2] * y3, code<L>(isamp, 3));
         }
     }
 
@@ -141,16 +165,23 @@ struct PfLane {
         }
     }
 
+    template<int L = 1>
+    __device__ __forceinline__ void combine_from(float &V, unsigned int &C) const
+    {
+        if constexpr (L < S::NLEV) {
+            const bool g = bv[L] > V;
+            V = g ? bv[L] : V;
+            C = g ? token<L>(bc[L]) : C;
+            combine_from<L + 1>(V, C);
+        }
+    }
+
+    // Levels in order, strict '>': the first level with the bin's maximum; C is its token.
     __device__ __forceinline__ void combine(float &V, unsigned int &C) const
     {
         V = bv[0];
-        C = bc[0];
-#pragma unroll
-        for (int l = 1; l < S::NLEV; l++) {
-            const bool g = bv[l] > V;
-            V = g ? bv[l] : V;
-            C = g ? bc[l] : C;
-        }
+        C = token<0>(bc[0]);
+        combine_from<1>(V, C);
     }
 };
 

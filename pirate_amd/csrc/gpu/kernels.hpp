@@ -30,6 +30,7 @@ struct GridLdsArgs {
     long state_bstride;
     int npri, s;                         // s = (stage-1 dd_rank of p>=1) + 1
     int beam0;                           // global index of the batch's first beam
+    long chunk;                          // chunk index (selects the state parity)
 };
 long lds_state_nelts(int R, int npri, int s);   // per beam
 void launch_grid_lds(Dtype dt, const GridLdsArgs &a, hipStream_t st);

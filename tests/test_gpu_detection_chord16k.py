@@ -1,0 +1,77 @@
+"""Full-size property test on the bench workload (16 beams x 16384 channels, 300-1500 MHz,
+rank 14, 4 primary trees, T = 2048): a simulated pulse in each of beams 0-7, DMs 60-650
+spread over primary trees 0 (DM < ~370) and 1 (the sweep of a DM-650 pulse lasts ~14
+chunks), beams 8-15 noise only; unit-variance noise generated on the GPU. Each pulsed beam's
+strongest candidate must be its pulse, decoded to the injected DM (within 2%) and arrival
+time at the bottom of the band (within the tree's time resolution), with an SNR (analytic
+weights for the unit input variance) close to the injected 60; the noise-only beams must
+stay below 8 sigma."""
+
+import numpy as np
+import pytest
+
+from gpu_cases import CFG2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_chord16k_pulses_recovered():
+    import torch
+    import pirate_amd as pa
+    from pirate_amd import simpulse
+
+    cfg = pa.DedispersionConfig.from_dict(CFG2)
+    plan = pa.DedispersionPlan(cfg)
+    dd = pa.GpuDedisperser(plan)
+    dd.allocate()
+    dd.fill_analytic_weights(np.ones(plan.nfreq))      # unit-variance noise: out_max is an SNR
+    B, F, T = plan.beams_per_batch, plan.nfreq, plan.nt_in
+    edges = np.linspace(300.0, 1500.0, F + 1)
+    npulsed = B // 2
+    dms = np.linspace(60.0, 650.0, npulsed)
+    t_inf = 1.5 + 0.37 * np.arange(npulsed)    # seconds, spread over chunks 0-2
+    pulses = [simpulse.SinglePulse(dm=float(dms[b]), sm=0.0, intrinsic_width=3e-3, spectral_index=0.0,
+                                   undispersed_arrival_time_sec=float(t_inf[b]), time_sample_ms=1.0, snr=60.0,
+                                   freq_edges_MHz=edges, internal_nt=256) for b in range(npulsed)]
+    # per beam: flat (channel, sample, value) lists of the sparse pulse
+    sparse = []
+    for p in pulses:
+        ch = np.repeat(np.arange(F), p.freq_nt)
+        it = np.concatenate([np.arange(p.freq_it0[f], p.freq_it0[f] + p.freq_nt[f]) for f in range(F)])
+        sparse.append((ch, it, p.sparse_data))
+    nchunks = max(p.it_end for p in pulses) // T + 3
+
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(5)
+    best = [None] * B
+    for c in range(nchunks):
+        with dd.get_input(c) as arr:
+            arr.normal_(generator=gen)
+            for b, (ch, it, val) in enumerate(sparse):
+                m = (it >= c * T) & (it < (c + 1) * T)
+                if np.any(m):
+                    idx = (torch.from_numpy(ch[m]).cuda(), torch.from_numpy(it[m] - c * T).cuda())
+                    arr[b].index_put_(idx, torch.from_numpy(val[m]).cuda(), accumulate=True)
+        with dd.get_output(c) as out:
+            for itree in range(plan.ntrees):
+                mx = out.out_max[itree].cpu().numpy()
+                am = out.out_argmax[itree].cpu().numpy().view(np.uint32)
+                for b in range(B):
+                    i = np.unravel_index(np.argmax(mx[b]), mx[b].shape)
+                    if best[b] is None or mx[b][i] > best[b][0]:
+                        best[b] = (float(mx[b][i]), c, itree, int(i[0]), int(i[1]), int(am[b][i]))
+
+    for b in range(npulsed, B):
+        print(f'noise-only beam {b}: max {best[b][0]:.1f}')
+        assert best[b][0] < 8.0
+    for b in range(npulsed):
+        val, c, itree, idm, itime, tok = best[b]
+        fmin, fmax, tlo, thi, p = plan.decode_argmax(tok, itree, idm, itime)
+        _, _, dm_hat, ts, _ = plan.decode_argmax2(itree, fmin, fmax, tlo, thi, p)
+        t_expect = (t_inf[b] + simpulse.dispersion_delay(dms[b], 300.0)) / 1e-3 - c * T
+        tres = 2 ** plan.trees[itree].primary_tree_index
+        print(f'beam {b}: snr {val:.1f} tree {itree} dm {dm_hat:.1f} (injected {dms[b]:.1f}) '
+              f't {ts:.1f} (expected {t_expect:.1f})')
+        assert 0.6 * 60.0 < val < 1.15 * 60.0
+        assert abs(dm_hat - dms[b]) < 0.02 * dms[b] + 1.0
+        assert abs(ts - t_expect) < 4.0 * tres + 2.0
