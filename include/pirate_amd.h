@@ -89,6 +89,14 @@ int pdd_plan_decode_argmax2(const pdd_plan *plan, int64_t itree, int64_t fmin, i
 /* DedispersionPlan::compute_steady_state_it0(itree), DedispersionPlan.cpp:618-661. */
 int pdd_plan_steady_state_it0(const pdd_plan *plan, int64_t itree, int64_t *out, size_t n);
 
+/* PfAvarApproximation(plan, freq_variances).tree_variance[itree] and .per_tf[itree]
+ * (include/pirate/PfVariance.hpp:84-104, src_lib/PfVariance.cpp:238-366): analytic
+ * white-noise variances of tree itree's peak-finder outputs. freq_variances: float64
+ * (nfreq,), positive. tree_variance: float64 (N, ndm_wt, nprofiles). per_tf (may be NULL):
+ * float64 (2^pf_rank, ndm_wt, nprofiles), the frequency-summed pf-rank bands. */
+int pdd_plan_pf_variances(const pdd_plan *plan, const double *freq_variances, int64_t nfreq, int64_t itree,
+                          double *tree_variance, int64_t tv_nelts, double *per_tf, int64_t ptf_nelts);
+
 /* ---- GpuDedisperser (include/pirate/Dedisperser.hpp:45-392) ---- */
 
 /* GpuDedisperser::Params, Dedisperser.hpp:47-86 (stream_pool replaced by an internal
@@ -133,6 +141,11 @@ int pdd_release_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, vo
 /* GpuDedisperser::fill_all_weights(itree, weights), GpuDedisperser.cpp:1257-1279.
  * host_weights: float32 (nbatches_wt, beams_per_batch, ndm_wt, nt_wt, nprofiles, N), >= 0. */
 int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *host_weights, int64_t nelts);
+
+/* GpuDedisperser::fill_analytic_weights(freq_variances), Dedisperser.hpp:244,
+ * GpuDedisperser.cpp:1162-1238: weights 1/sqrt(analytic variance) for every tree, beam and
+ * weight slot, so out_max is an SNR. freq_variances: float64 (nfreq,), positive. */
+int pdd_fill_analytic_weights(pdd_dedisperser *dd, const double *freq_variances, int64_t nfreq);
 
 /* GpuDedisperser::stop(e), GpuDedisperser.cpp:520-577. */
 int pdd_stop(pdd_dedisperser *dd, const char *reason);

@@ -8,6 +8,7 @@ over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library wit
 from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
 from .dedisperser import GpuDedisperser, GpuDedisperserOutputs
 from .kernels import GpuDequantizationKernel
+from .pf_variance import PfAvarApproximation
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
-           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel']
+           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel', 'PfAvarApproximation']

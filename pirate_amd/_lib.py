@@ -49,6 +49,8 @@ _SIGS = {
     'pdd_plan_decode_argmax': (ctypes.c_int, [vp, ctypes.c_uint32, i64, i64, i64, ctypes.POINTER(i64)]),
     'pdd_plan_decode_argmax2': (ctypes.c_int, [vp, i64, i64, i64, i64, i64, i64, ctypes.POINTER(ctypes.c_double)]),
     'pdd_plan_steady_state_it0': (ctypes.c_int, [vp, i64, ctypes.POINTER(i64), ctypes.c_size_t]),
+    'pdd_plan_pf_variances': (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), i64, i64, ctypes.POINTER(ctypes.c_double), i64,
+                                             ctypes.POINTER(ctypes.c_double), i64]),
     'pdd_dedisperser_create': (ctypes.c_int, [vp, ctypes.POINTER(DedisperserParams), ctypes.POINTER(vp)]),
     'pdd_dedisperser_destroy': (None, [vp]),
     'pdd_dedisperser_allocate': (ctypes.c_int, [vp]),
@@ -59,6 +61,7 @@ _SIGS = {
     'pdd_release_output': (ctypes.c_int, [vp, i64, i64, vp]),
     'pdd_dequantize': (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, vp, vp, vp, vp]),
     'pdd_fill_all_weights': (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_float), i64]),
+    'pdd_fill_analytic_weights': (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), i64]),
     'pdd_stop': (ctypes.c_int, [vp, ctypes.c_char_p]),
     'pdd_kernel_timing': (ctypes.c_int, [vp, ctypes.c_int]),
     'pdd_kernel_timing_report': (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),

@@ -4,6 +4,7 @@
 #include "host/common.hpp"
 #include "host/config.hpp"
 #include "host/plan.hpp"
+#include "host/pf_variance.hpp"
 #include "gpu/dedisperser.hpp"
 #include "gpu/kernels.hpp"
 
@@ -199,6 +200,24 @@ int pdd_plan_steady_state_it0(const pdd_plan *p, int64_t itree, int64_t *out, si
     });
 }
 
+int pdd_plan_pf_variances(const pdd_plan *p, const double *freq_variances, int64_t nfreq, int64_t itree,
+                          double *tree_variance, int64_t tv_nelts, double *per_tf, int64_t ptf_nelts)
+{
+    PDD_TRY({
+        PDD_CHECK(p && freq_variances && tree_variance, "null argument");
+        PDD_CHECK(itree >= 0 && itree < p->p->ntrees, "pdd_plan_pf_variances: itree out of range");
+        PfAvarApproximation avar(*p->p, freq_variances, nfreq, itree);
+        const std::vector<double> &tv = avar.tree_variance[itree];
+        const std::vector<double> &pt = avar.per_tf[itree];
+        PDD_CHECK(tv_nelts == int64_t(tv.size()), "pdd_plan_pf_variances: tree_variance needs " << tv.size() << " elements (N, ndm_wt, nprofiles), got " << tv_nelts);
+        std::copy(tv.begin(), tv.end(), tree_variance);
+        if (per_tf) {
+            PDD_CHECK(ptf_nelts == int64_t(pt.size()), "pdd_plan_pf_variances: per_tf needs " << pt.size() << " elements (2^pf_rank, ndm_wt, nprofiles), got " << ptf_nelts);
+            std::copy(pt.begin(), pt.end(), per_tf);
+        }
+    });
+}
+
 int pdd_dedisperser_create(const pdd_plan *p, const pdd_dedisperser_params *prm, pdd_dedisperser **out)
 {
     PDD_TRY({
@@ -264,6 +283,14 @@ int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *w, int
     PDD_TRY_DD(dd, {
         PDD_CHECK(w, "null weights");
         dd->d->fill_all_weights(itree, w, nelts);
+    });
+}
+
+int pdd_fill_analytic_weights(pdd_dedisperser *dd, const double *freq_variances, int64_t nfreq)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(freq_variances, "null freq_variances");
+        dd->d->fill_analytic_weights(freq_variances, nfreq);
     });
 }
 

@@ -423,6 +423,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         g.npri = int(npri);
         g.s = (npri > 1) ? int(plan->stage1_dd_rank[1] + 1) : 1;
         g.beam0 = beam0;
+        g.chunk = ichunk;
         hipEvent_t ev;
         timed_begin("grid_lds", st, ev);
         launch_grid_lds(dtype, g, st);
@@ -697,6 +698,39 @@ void GpuDedisperser::fill_all_weights(long itree, const float *host, long nelts)
     for (long i = 0; i < nelts; i++)
         PDD_CHECK(std::isfinite(host[i]) && host[i] >= 0.0f, "fill_all_weights(): weights must be finite and >= 0 (index " << i << ")");
     HIP_CHECK(hipMemcpy(weights[itree].ptr, host, nelts * sizeof(float), hipMemcpyHostToDevice));
+}
+
+
+// Reference: GpuDedisperser::fill_analytic_weights (src_lib/GpuDedisperser.cpp:1162-1238) and
+// PeakFindingKernelParams::fill_host_weights with variances (src_lib/PeakFindingKernel.cu:
+// 606-691, randomize=false): w[b, d, t, p, n] = 1/sqrt(float(variance[n, d, p])) in every
+// beam and weight slot.
+void GpuDedisperser::fill_analytic_weights(const double *freq_variances, long nfreq_in)
+{
+    {
+        std::lock_guard<std::mutex> lk(mutex);
+        check_stopped_locked();
+        PDD_CHECK(is_allocated, "fill_analytic_weights(): allocate() has not been called");
+    }
+    const PfAvarApproximation avar(*plan, freq_variances, nfreq_in);
+    for (long itree = 0; itree < ntrees; itree++) {
+        const DedispersionTree &t = plan->trees[itree];
+        const long D = t.ndm_wt, T = t.nt_wt, P = t.nprofiles, N = t.frequency_subbands.N;
+        const std::vector<double> &v = avar.tree_variance[itree];     // (N, D, P)
+        std::vector<float> base(D * P * N);                            // (D, P, N)
+        for (long n = 0; n < N; n++)
+            for (long d = 0; d < D; d++)
+                for (long p = 0; p < P; p++)
+                    base[(d * P + p) * N + n] = 1.0f / std::sqrt(float(v[(n * D + d) * P + p]));
+        const long nslot = nbatches_wt * beams_per_batch;
+        std::vector<float> host(nslot * D * T * P * N);
+        float *o = host.data();
+        for (long s = 0; s < nslot; s++)
+            for (long d = 0; d < D; d++)
+                for (long it = 0; it < T; it++, o += P * N)
+                    std::memcpy(o, &base[d * P * N], P * N * sizeof(float));
+        fill_all_weights(itree, host.data(), long(host.size()));
+    }
 }
 
 

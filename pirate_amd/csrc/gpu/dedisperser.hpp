@@ -67,6 +67,11 @@ public:
     // host: (nbatches_wt, B, ndm_wt, nt_wt, P, N) float32, all entries finite and >= 0.
     void fill_all_weights(long itree, const float *host, long nelts);
 
+    // Weights 1/sqrt(analytic variance) for every tree and weight slot, so that the peak
+    // finder's output is an SNR for white-noise input with these per-channel variances
+    // (length nfreq, positive).
+    void fill_analytic_weights(const double *freq_variances, long nfreq);
+
     void stop(const std::string &reason);
     bool is_stopped();
 

@@ -194,6 +194,16 @@ class GpuDedisperser:
             raise ValueError(f'fill_all_weights: expected shape {self.wt_shape(itree)}, got {w.shape}')
         L.check(L.lib().pdd_fill_all_weights(self._h, itree, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size))
 
+    def fill_analytic_weights(self, freq_variances):
+        """Weights 1/sqrt(analytic variance) for every tree, beam and weight slot, so that
+        out_max is an SNR for white-noise input with these per-channel variances
+        (float64, (nfreq,), positive). Reference: GpuDedisperser::fill_analytic_weights
+        (include/pirate/Dedisperser.hpp:244, src_lib/GpuDedisperser.cpp:1162-1238)."""
+        fv = np.ascontiguousarray(freq_variances, dtype=np.float64)
+        if fv.shape != (self.nfreq,):
+            raise ValueError(f'fill_analytic_weights: freq_variances must have shape ({self.nfreq},), got {fv.shape}')
+        L.check(L.lib().pdd_fill_analytic_weights(self._h, fv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fv.size))
+
     def fill_host_weights(self, itree, variances=None, randomize=False, rng=None):
         """Builds and uploads tree itree's weights the way the reference's
         PeakFindingKernelParams::fill_host_weights does (src_lib/PeakFindingKernel.cu:588-691):
@@ -206,7 +216,9 @@ class GpuDedisperser:
         t = self.trees[itree]
         nb, B, D, T, P, N = self.wt_shape(itree)
         if variances is None:
-            base = np.broadcast_to((1.0 / np.sqrt(bare_profile_variances(P))).astype(np.float32)[None, None, :], (D, N, P))
+            # rsqrtf(float(var)) as in the reference: the square root is taken in float32
+            bw = np.float32(1.0) / np.sqrt(bare_profile_variances(P).astype(np.float32))
+            base = np.broadcast_to(bw[None, None, :], (D, N, P))
         else:
             v = np.asarray(variances, dtype=np.float64)
             if v.shape != (N, D, P):

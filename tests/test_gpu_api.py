@@ -121,3 +121,37 @@ def test_fill_host_weights_bare_and_variances():
         assert np.array_equal(w[0, 0, :, 0], want)
         w = dd.fill_host_weights(it, var, randomize=True, rng=np.random.default_rng(1))
         assert np.all(w >= 0) and np.all(w <= np.broadcast_to(want[None, None, :, None], w.shape))
+
+
+def test_fill_analytic_weights_equals_explicit_variance_weights():
+    """fill_analytic_weights(fv) uploads 1/sqrt(PfAvarApproximation(plan, fv).tree_variance)
+    to every beam and weight slot (src_lib/GpuDedisperser.cpp:1162-1238): the chain's outputs
+    equal those with the same weights built on the host and uploaded by fill_all_weights."""
+    import torch
+    import pirate_amd as pa
+    a, b = make('toy'), make('toy')
+    fv = np.random.default_rng(4).uniform(0.5, 2.0, a.nfreq)
+    a.fill_analytic_weights(fv)
+    av = pa.PfAvarApproximation(a.plan, fv)
+    for it in range(b.ntrees):
+        w = b.fill_host_weights(it, av.tree_variance[it])
+        assert np.all(w > 0)
+    gen = torch.Generator(device='cuda')
+    for seq in range(3 * a.nbatches):
+        gen.manual_seed(seq)
+        x = torch.randn((a.beams_per_batch, a.nfreq, a.nt_in), generator=gen, device='cuda')
+        for dd in (a, b):
+            with dd.get_input(seq) as arr:
+                arr.copy_(x)
+        with a.get_output(seq) as oa, b.get_output(seq) as ob:
+            for it in range(a.ntrees):
+                assert torch.equal(oa.out_max[it], ob.out_max[it])
+                assert torch.equal(oa.out_argmax[it], ob.out_argmax[it])
+
+
+def test_fill_analytic_weights_bad_variances_stop():
+    dd = make('toy')
+    with pytest.raises(RuntimeError, match='positive'):
+        dd.fill_analytic_weights(np.zeros(dd.nfreq))
+    with pytest.raises(RuntimeError, match='stopped'):
+        dd._acquire_input(0, 0)
