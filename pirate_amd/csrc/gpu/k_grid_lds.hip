@@ -9,24 +9,29 @@
 //     out_p[j][t'] = c*(ds_p[2j][t-1] + ds_p[2j+1][t]),  t' = t + L_j     (one-sample lag on
 //                    the even row, then the "large" lag L_j = 2^(s-1) - 1 - (j mod 2^(s-1)))
 //
-// Mapping: one wave per (tree-channel pair (2j, 2j+1), time tile of 64 lanes x G samples),
+// Mapping: one wave per (tree-channel pair (2j, 2j+1), time tile of 64 x G samples),
 // G = max(4, 2^(npri-1)); a workgroup holds 4 consecutive tiles of one pair. Waves never
-// wait on each other (no LDS, no barriers). Lane l owns G consecutive samples of both rows:
-// it grids them from the input channels (16-byte loads), stores the gridded rows (the input
-// of primary tree 0), and keeps them in registers for the downsampling levels, so the
-// gridded chunk is never read back from HBM. The one-sample lag crosses lanes with a
-// shuffle; lane 0 recomputes the previous group of the even row itself (for the first tile:
-// the previous chunk's last value, from per-beam state). The large lag is a write offset:
-// outputs past the chunk end go to per-beam state and are copied to the next chunk's first
-// L_j samples. The state is double-buffered by chunk parity, so the first tile's reads and
-// the last tile's writes of one launch never touch the same memory.
+// wait on each other (no barriers; LDS is private to each wave). The wave grids its tile in
+// a coalesced layout (lane l: 4 consecutive samples in each 256-sample block, so every
+// 16-byte load and store instruction covers 1 KB of a row), stores the gridded rows (the
+// input of primary tree 0), and transposes them through LDS to G consecutive samples per
+// lane for the downsampling levels, so the gridded chunk is never read back from HBM. The
+// one-sample lag crosses lanes with a shuffle; lane 0 recomputes the previous group of the
+// even row itself (for the first tile: the previous chunk's last value, from per-beam
+// state). The large lag is a write offset: outputs past the chunk end go to per-beam state
+// and are copied to the next chunk's first L_j samples. The state is double-buffered by
+// chunk parity, so the first tile's reads and the last tile's writes of one launch never
+// touch the same memory.
 // With 16-bit storage the gridded values are rounded to the storage type before the
 // downsampling levels use them (as if stored and re-read); the levels themselves stay in
 // float32 and round only at their output stores.
 //
-// Workgroups are mapped to channel ranges XCD by XCD (workgroup i runs on XCD i mod 8), so
-// neighbouring channel pairs, which share boundary input channels, run on the same XCD and
-// share its L2.
+// Workgroups run in launch order (consecutive workgroups go to the 8 XCDs in turn), so at any
+// moment the device streams one contiguous range of channels of one beam. Neighbouring
+// channel pairs share boundary input channels; a second read of such a channel from another
+// XCD is served by the device-wide Infinity Cache. (Giving each XCD a contiguous eighth of
+// the channels instead, for L2 sharing, made the device stream eight distant regions at once
+// and measured 44% slower on MI355X.)
 
 #include <algorithm>
 #include <type_traits>
@@ -59,12 +64,13 @@ __device__ __forceinline__ float quant(float x)
     }
 }
 
-// Grids tree channels 2j ("e") and 2j+1 ("o") for G samples at 'in'. The two channels'
-// input ranges [fe, fe+ce) and [fo, fo+co) are adjacent or overlapping: each channel of
-// their union is loaded once, in batches of 4 whose loads are all issued before use, and
-// added to each row whose range contains it, in ascending channel order.
-template<typename Tst, int G>
-__device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int fe, int ce, const float *we,
+// Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/4 groups of 4 consecutive
+// samples at in + QS*q (q < nq; groups q >= nq are outside the chunk and left at zero). The
+// two channels' input ranges [fe, fe+ce) and [fo, fo+co) are adjacent or overlapping: each
+// channel of their union is loaded once, in batches of 4 whose loads are all issued before
+// use, and added to each row whose range contains it, in ascending channel order.
+template<typename Tst, int G, int QS>
+__device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int nq, int fe, int ce, const float *we,
                                           int fo, int co, const float *wo, float (&xe)[G], float (&xo)[G])
 {
 #pragma unroll
@@ -78,7 +84,7 @@ __device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int fe, int
             const Tst *p = in + long(min(f1 + u, fb - 1)) * ntime;
 #pragma unroll
             for (int q = 0; q < G / 4; q++)
-                v[u][q] = ld4_st(p + 4 * q);
+                v[u][q] = (q < nq) ? ld4_st(p + QS * q) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -120,9 +126,7 @@ k_grid_lds(GridLdsArgs a)
     const int ngroups = a.ntime / G;
     const int ntiles = (ngroups + 63) / 64;
     const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;          // workgroups per channel pair
-    const int nblk = gridDim.x;
-    const int xb = blockIdx.x;
-    const int blk = (nblk % 8 == 0) ? ((xb % 8) * (nblk / 8) + xb / 8) : xb;
+    const int blk = blockIdx.x;
     const int j = blk / tpb;                                       // channel pair
     const int tile = (blk - j * tpb) * kGlWaves + (threadIdx.x >> 6);
     const int b = blockIdx.y;
@@ -141,21 +145,21 @@ k_grid_lds(GridLdsArgs a)
     const int g0 = tile * 64;
     const int g = g0 + lane;
     const int nact = min(64, ngroups - g0);
-    const bool active = (lane < nact);
     const int t0 = g * G;
 
+    // Gridding in the coalesced layout: lane l holds samples tb0 + 256 q + 4 l + (0..3),
+    // q < G/4, so that each 16-byte load and store instruction of the wave covers 1 KB of a
+    // row. Groups past the chunk end (a partial last tile) are skipped.
+    const int tb0 = g0 * G;
+    const int nq = min(G / 4, max(0, (a.ntime - tb0 - 4 * lane + 255) >> 8));
     float xe[G], xo[G];
-    if (active) {
-        grid_pair<Tst, G>(in + t0, a.ntime, fe, ce, we, fo, co, wo, xe, xo);
+    grid_pair<Tst, G, 256>(in + tb0 + 4 * lane, a.ntime, nq, fe, ce, we, fo, co, wo, xe, xo);
 #pragma unroll
-        for (int q = 0; q < G; q += 4) {
-            st4_st(gout_e + t0 + q, make_float4(xe[q], xe[q + 1], xe[q + 2], xe[q + 3]));
-            st4_st(gout_o + t0 + q, make_float4(xo[q], xo[q + 1], xo[q + 2], xo[q + 3]));
+    for (int q = 0; q < G / 4; q++) {
+        if (q < nq) {
+            st4_st(gout_e + tb0 + 256 * q + 4 * lane, make_float4(xe[4 * q], xe[4 * q + 1], xe[4 * q + 2], xe[4 * q + 3]));
+            st4_st(gout_o + tb0 + 256 * q + 4 * lane, make_float4(xo[4 * q], xo[4 * q + 1], xo[4 * q + 2], xo[4 * q + 3]));
         }
-    } else {
-#pragma unroll
-        for (int i = 0; i < G; i++)
-            xe[i] = xo[i] = 0.0f;
     }
 
     if constexpr (NPRI > 1) {
@@ -186,7 +190,7 @@ k_grid_lds(GridLdsArgs a)
                     prevs[p] = rd_small[long(p - 1) * nrows + j];
             } else {
                 float xp[G], xq[G];
-                grid_pair<Tst, G>(in + t0 - G, a.ntime, fe, ce, we, fe, 0, we, xp, xq);
+                grid_pair<Tst, G, 4>(in + t0 - G, a.ntime, G / 4, fe, ce, we, fe, 0, we, xp, xq);
 #pragma unroll
                 for (int i = 0; i < G; i++)
                     xp[i] = quant<Tst>(xp[i]);
@@ -211,16 +215,35 @@ k_grid_lds(GridLdsArgs a)
             }
         }
 
+        // Per-wave LDS transpose buffer. First the gridded rows go from the coalesced layout
+        // to G consecutive samples per lane (lane l: samples tb0 + G l + i), the layout of the
+        // downsampling levels; then each level's outputs go back so that each global store
+        // instruction writes 64 consecutive samples.
+        __shared__ float tbuf[kGlWaves][64 * G];
+        float *tb = tbuf[threadIdx.x >> 6];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            float (&x)[G] = r ? xo : xe;
+#pragma unroll
+            for (int q = 0; q < G / 4; q++)
+                *reinterpret_cast<float4 *>(tb + 256 * q + 4 * lane) = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < G / 4; q++) {
+                const float4 y = *reinterpret_cast<const float4 *>(tb + G * lane + 4 * q);
+                x[4 * q] = y.x;
+                x[4 * q + 1] = y.y;
+                x[4 * q + 2] = y.z;
+                x[4 * q + 3] = y.w;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
 #pragma unroll
         for (int i = 0; i < G; i++) {
             xe[i] = quant<Tst>(xe[i]);
             xo[i] = quant<Tst>(xo[i]);
         }
         const bool last_tile = (tile == ntiles - 1);
-        // Per-wave LDS transpose buffer: a lane's n consecutive outputs are written to LDS and
-        // read back so that each global store instruction writes 64 consecutive samples.
-        __shared__ float tbuf[kGlWaves][64 * (G / 2)];
-        float *tb = tbuf[threadIdx.x >> 6];
 #pragma unroll
         for (int p = 1; p < NPRI; p++) {
             const int n = G >> p;
