@@ -26,6 +26,7 @@ extern "C" {
 typedef struct pdd_config pdd_config;
 typedef struct pdd_plan pdd_plan;
 typedef struct pdd_dedisperser pdd_dedisperser;
+typedef struct pdd_ringbuf_copy pdd_ringbuf_copy;
 
 /* Last error message of the calling thread ("" if none). */
 const char *pdd_last_error(void);
@@ -161,6 +162,27 @@ int pdd_stop(pdd_dedisperser *dd, const char *reason);
  * rounded to 16 bits for 16-bit outputs. Asynchronous on 'stream'. */
 int pdd_dequantize(const char *out_dtype, int64_t nbeams, int64_t nfreq, int64_t ntime, const void *data,
                    const void *scales_offsets, void *out, void *stream);
+
+/* ---- ring-buffer segment copies ---- */
+
+/* GpuRingbufCopyKernel(params) + allocate() (include/pirate/RingbufCopyKernel.hpp:15-60,
+ * src_lib/RingbufCopyKernel.cu:23-45, 128-158). octuples: host array of 8*N uint32, N rows of
+ * {src_offset, src_phase, src_len, src_nseg, dst_offset, dst_phase, dst_len, dst_nseg}
+ * (offsets in segments; len = frames in the zone; nseg = segments per frame), copied to the
+ * device. nbytes_per_segment: 128 or 256 (nelts_per_segment x element size).
+ * total_beams must be a multiple of beams_per_batch. */
+int pdd_ringbuf_copy_create(int64_t total_beams, int64_t beams_per_batch, int64_t nbytes_per_segment,
+                            const uint32_t *octuples, int64_t n_uint32, pdd_ringbuf_copy **out);
+void pdd_ringbuf_copy_destroy(pdd_ringbuf_copy *k);
+
+/* GpuRingbufCopyKernel::launch(ringbuf, ichunk, ibatch, stream), RingbufCopyKernel.cu:197-220
+ * (semantics of CpuRingbufCopyKernel::apply, :60-123): for iframe = ichunk * total_beams +
+ * ibatch * beams_per_batch and beams b < beams_per_batch, copies segment
+ * src_offset + ((src_phase + iframe + b) % src_len) * src_nseg to the same formula on the dst
+ * side. ringbuf: device pointer (16-byte aligned) to ringbuf_nbytes bytes; every addressed
+ * segment is checked against ringbuf_nbytes before launch. Asynchronous on 'stream'. */
+int pdd_ringbuf_copy_launch(pdd_ringbuf_copy *k, void *ringbuf, int64_t ringbuf_nbytes, int64_t ichunk,
+                            int64_t ibatch, void *stream);
 
 /* Kernel timing (reference counterpart: the per-kernel KernelTimer loops and ResourceTracker
  * rates of GpuDedisperser::time, src_lib/GpuDedisperser.cpp:1573-1690). When enabled, HIP

@@ -328,3 +328,18 @@ extern "C" int orc_dequantize(const uint8_t *data, const uint16_t *scoff, long n
     ref_dequantize(data, scoff, nbeams, nfreq, ntime, out);
     return 0;
 }
+
+extern "C" int orc_ringbuf_copy(uint8_t *ringbuf, long nbytes, const uint32_t *oct, long noct, long seg_bytes,
+                                long nbeams, unsigned long long iframe)
+{
+    for (long i = 0; i < noct; i++)
+        for (int side = 0; side < 2; side++) {
+            const uint32_t *q = oct + 8 * i + 4 * side;
+            if (q[2] == 0 || (long(q[0]) + long(q[2] - 1) * long(q[3]) + 1) * seg_bytes > nbytes) {
+                g_err = "orc_ringbuf_copy: octuple addresses a segment outside the buffer";
+                return 1;
+            }
+        }
+    ref_ringbuf_copy(ringbuf, oct, noct, seg_bytes, nbeams, iframe);
+    return 0;
+}

@@ -914,6 +914,30 @@ struct RefDedisperser {
 
 }  // namespace oracle
 
+// -------------------------------------------------------------------------------------------
+// Ring-buffer segment copies driven by octuples (reference: CpuRingbufCopyKernel::apply and its
+// helper, src_lib/RingbufCopyKernel.cu:60-123). Octuple i = {src_offset, src_phase, src_len,
+// src_nseg, dst_offset, dst_phase, dst_len, dst_nseg}; for beams b = 0..nbeams-1 in order,
+// segment src_offset + ((src_phase + iframe + b) % src_len) * src_nseg is copied (memmove) to
+// the same formula on the dst side. Octuples are processed in order.
+
+inline void ref_ringbuf_copy(uint8_t *ringbuf, const uint32_t *oct, long noct, long seg_bytes, long nbeams,
+                             uint64_t iframe)
+{
+    for (long i = 0; i < noct; i++) {
+        const uint32_t *o = oct + 8 * i;
+        uint64_t sph = (uint64_t(o[1]) + iframe) % uint64_t(o[2]);
+        uint64_t dph = (uint64_t(o[5]) + iframe) % uint64_t(o[6]);
+        for (long b = 0; b < nbeams; b++) {
+            const uint64_t s = (uint64_t(o[0]) + sph * uint64_t(o[3])) * uint64_t(seg_bytes);
+            const uint64_t d = (uint64_t(o[4]) + dph * uint64_t(o[7])) * uint64_t(seg_bytes);
+            std::memmove(ringbuf + d, ringbuf + s, size_t(seg_bytes));
+            sph = (sph + 1 == o[2]) ? 0 : sph + 1;
+            dph = (dph + 1 == o[6]) ? 0 : dph + 1;
+        }
+    }
+}
+
 #endif
 
 

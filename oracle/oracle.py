@@ -371,3 +371,13 @@ def dequantize(data_u8, scoff_f16, nbeams, nfreq, ntime):
     _check(lib().orc_dequantize(_ptr(d), _ptr(s), ctypes.c_long(nbeams), ctypes.c_long(nfreq),
                                 ctypes.c_long(ntime), _ptr(out)))
     return out
+
+
+def ringbuf_copy(ringbuf_u8, octuples, seg_bytes, nbeams, iframe):
+    """Reference ring-buffer copies (CpuRingbufCopyKernel::apply, src_lib/RingbufCopyKernel.cu:
+    60-123), in place on a uint8 array. octuples: uint32 (N, 8)."""
+    assert ringbuf_u8.dtype == np.uint8 and ringbuf_u8.flags.c_contiguous
+    o = np.ascontiguousarray(octuples, dtype=np.uint32).reshape(-1, 8)
+    _check(lib().orc_ringbuf_copy(_ptr(ringbuf_u8), ctypes.c_long(ringbuf_u8.size), _ptr(o), ctypes.c_long(len(o)),
+                                  ctypes.c_long(seg_bytes), ctypes.c_long(nbeams), ctypes.c_ulonglong(iframe)))
+    return ringbuf_u8

@@ -60,6 +60,9 @@ _SIGS = {
     'pdd_acquire_output': (ctypes.c_int, [vp, i64, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),
     'pdd_release_output': (ctypes.c_int, [vp, i64, i64, vp]),
     'pdd_dequantize': (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, vp, vp, vp, vp]),
+    'pdd_ringbuf_copy_create': (ctypes.c_int, [i64, i64, i64, ctypes.POINTER(ctypes.c_uint32), i64, ctypes.POINTER(vp)]),
+    'pdd_ringbuf_copy_destroy': (None, [vp]),
+    'pdd_ringbuf_copy_launch': (ctypes.c_int, [vp, vp, i64, i64, i64, vp]),
     'pdd_fill_all_weights': (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_float), i64]),
     'pdd_fill_analytic_weights': (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), i64]),
     'pdd_stop': (ctypes.c_int, [vp, ctypes.c_char_p]),

@@ -8,6 +8,7 @@
 #include "gpu/dedisperser.hpp"
 #include "gpu/kernels.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -336,6 +337,78 @@ int pdd_dequantize(const char *out_dtype, int64_t nbeams, int64_t nfreq, int64_t
         launch_dequant(dt, data, scales_offsets, out, nbeams * nfreq * ntime, (hipStream_t) stream);
         hipError_t e = hipGetLastError();
         PDD_CHECK(e == hipSuccess, "pdd_dequantize: launch failed: " << hipGetErrorString(e));
+    });
+}
+
+static void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        throw pdd_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+}
+
+struct pdd_ringbuf_copy {
+    long total_beams = 0, beams_per_batch = 0, seg_bytes = 0, noct = 0;
+    long max_seg = -1;             // largest segment index any octuple addresses
+    uint32_t *d_oct = nullptr;     // device copy of the octuples
+};
+
+int pdd_ringbuf_copy_create(int64_t total_beams, int64_t beams_per_batch, int64_t nbytes_per_segment,
+                            const uint32_t *octuples, int64_t n_uint32, pdd_ringbuf_copy **out)
+{
+    PDD_TRY({
+        PDD_CHECK(out, "pdd_ringbuf_copy_create: null output handle");
+        PDD_CHECK(total_beams > 0 && beams_per_batch > 0 && total_beams % beams_per_batch == 0,
+                  "pdd_ringbuf_copy_create: total_beams must be a positive multiple of beams_per_batch");
+        PDD_CHECK(nbytes_per_segment == 128 || nbytes_per_segment == 256,
+                  "pdd_ringbuf_copy_create: nbytes_per_segment must be 128 or 256 (got " << nbytes_per_segment << ")");
+        PDD_CHECK(n_uint32 >= 0 && n_uint32 % 8 == 0, "pdd_ringbuf_copy_create: octuples must hold 8*N uint32");
+        PDD_CHECK(n_uint32 == 0 || octuples, "pdd_ringbuf_copy_create: null octuples");
+        std::unique_ptr<pdd_ringbuf_copy> k(new pdd_ringbuf_copy);
+        k->total_beams = total_beams;
+        k->beams_per_batch = beams_per_batch;
+        k->seg_bytes = nbytes_per_segment;
+        k->noct = n_uint32 / 8;
+        for (long i = 0; i < k->noct; i++)
+            for (int side = 0; side < 2; side++) {
+                const uint32_t *q = octuples + 8 * i + 4 * side;
+                PDD_CHECK(q[2] > 0, "pdd_ringbuf_copy_create: octuple " << i << " has a zero frame count");
+                k->max_seg = std::max(k->max_seg, long(q[0]) + long(q[2] - 1) * long(q[3]));
+            }
+        if (k->noct > 0) {
+            hip_check(hipMalloc(&k->d_oct, n_uint32 * sizeof(uint32_t)), "hipMalloc(octuples)");
+            const hipError_t e = hipMemcpy(k->d_oct, octuples, n_uint32 * sizeof(uint32_t), hipMemcpyHostToDevice);
+            if (e != hipSuccess)
+                (void) hipFree(k->d_oct);
+            hip_check(e, "hipMemcpy(octuples)");
+        }
+        *out = k.release();
+    });
+}
+
+void pdd_ringbuf_copy_destroy(pdd_ringbuf_copy *k)
+{
+    if (k) {
+        if (k->d_oct)
+            (void) hipFree(k->d_oct);
+        delete k;
+    }
+}
+
+int pdd_ringbuf_copy_launch(pdd_ringbuf_copy *k, void *ringbuf, int64_t ringbuf_nbytes, int64_t ichunk,
+                            int64_t ibatch, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(k && ringbuf, "pdd_ringbuf_copy_launch: null argument");
+        PDD_CHECK((reinterpret_cast<uintptr_t>(ringbuf) % 16) == 0, "pdd_ringbuf_copy_launch: ringbuf must be 16-byte aligned");
+        PDD_CHECK(ichunk >= 0 && ibatch >= 0 && ibatch * k->beams_per_batch < k->total_beams,
+                  "pdd_ringbuf_copy_launch: bad (ichunk, ibatch) = (" << ichunk << ", " << ibatch << ")");
+        PDD_CHECK((k->max_seg + 1) * k->seg_bytes <= ringbuf_nbytes,
+                  "pdd_ringbuf_copy_launch: octuples address segment " << k->max_seg << ", beyond the "
+                  << ringbuf_nbytes << "-byte ring buffer");
+        const unsigned long long iframe = (unsigned long long) (ichunk * k->total_beams + ibatch * k->beams_per_batch);
+        launch_ringbuf_copy(ringbuf, k->d_oct, k->noct, k->seg_bytes, int(k->beams_per_batch), iframe, (hipStream_t) stream);
+        hipError_t e = hipGetLastError();
+        PDD_CHECK(e == hipSuccess, "pdd_ringbuf_copy_launch: launch failed: " << hipGetErrorString(e));
     });
 }
 

@@ -96,6 +96,11 @@ void launch_stage2_fast(Dtype dt, const Stage2Args &a, hipStream_t st);
 void launch_dequant(Dtype dt, const void *data, const void *scoff, void *out, long nsamples, hipStream_t st);
 
 
+// ---- ring-buffer segment copies driven by octuples (device array of 8*noct uint32) ----
+void launch_ringbuf_copy(void *ringbuf, const uint32_t *octuples, long noct, long seg_bytes, int nbeams,
+                         unsigned long long iframe, hipStream_t st);
+
+
 // ---- peak finder over (B, ndm_out, M, rowlen) rows -> out_max, out_argmax ----
 struct PfArgs {
     void *sb;                 // rows [tpad history | Tp | future (max final lag)]
