@@ -7,8 +7,10 @@ over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library wit
 
 from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
 from .dedisperser import GpuDedisperser, GpuDedisperserOutputs
-from .kernels import GpuDequantizationKernel
+from .kernels import GpuDequantizationKernel, GpuRingbufCopyKernel
 from .pf_variance import PfAvarApproximation
+from .offline import AssembledFrame, OfflineDedisperser, XEngineMetadata
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
-           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel', 'PfAvarApproximation']
+           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
+           'PfAvarApproximation', 'OfflineDedisperser', 'AssembledFrame', 'XEngineMetadata']
