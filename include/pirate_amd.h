@@ -82,6 +82,13 @@ int pdd_plan_tree_info(const pdd_plan *plan, int64_t itree, pdd_tree_info *out);
 int pdd_plan_decode_argmax(const pdd_plan *plan, uint32_t token, int64_t itree, int64_t idm_coarse,
                            int64_t itime_coarse, int64_t *out5);
 
+/* Batch form of decode_argmax (the vectorized decoding the reference's Python binding offers,
+ * src_pybind11/pirate_pybind11.cpp:356-415): tokens is an array of n uint32 whose last two axes
+ * are (ndm_out, nt_out) of tree itree (n a multiple of ndm_out * nt_out); element i is decoded
+ * at idm_coarse = (i / nt_out) % ndm_out, itime_coarse = i % nt_out into out[5*i .. 5*i+4]
+ * = {fmin, fmax, tlo, thi, p}. */
+int pdd_plan_decode_argmax_batch(const pdd_plan *plan, int64_t itree, const uint32_t *tokens, int64_t n, int64_t *out);
+
 /* DedispersionPlan::decode_argmax2(), DedispersionPlan.cpp:551-615.
  * out5 = {freq_lo_MHz, freq_hi_MHz, dm, timestamp_samp, width_samp}. */
 int pdd_plan_decode_argmax2(const pdd_plan *plan, int64_t itree, int64_t fmin, int64_t fmax,

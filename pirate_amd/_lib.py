@@ -47,6 +47,7 @@ _SIGS = {
     'pdd_plan_ntrees': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
     'pdd_plan_tree_info': (ctypes.c_int, [vp, i64, ctypes.POINTER(TreeInfo)]),
     'pdd_plan_decode_argmax': (ctypes.c_int, [vp, ctypes.c_uint32, i64, i64, i64, ctypes.POINTER(i64)]),
+    'pdd_plan_decode_argmax_batch': (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_uint32), i64, ctypes.POINTER(i64)]),
     'pdd_plan_decode_argmax2': (ctypes.c_int, [vp, i64, i64, i64, i64, i64, i64, ctypes.POINTER(ctypes.c_double)]),
     'pdd_plan_steady_state_it0': (ctypes.c_int, [vp, i64, ctypes.POINTER(i64), ctypes.c_size_t]),
     'pdd_plan_pf_variances': (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), i64, i64, ctypes.POINTER(ctypes.c_double), i64,

@@ -246,17 +246,18 @@ class DedispersionPlan:
         return tuple(float(x) for x in out)
 
     def decode_argmax_batch(self, itree, out_argmax):
-        """Vector version of decode_argmax over an (..., ndm_out, nt_out) token array.
-        Returns int64 array of shape (..., ndm_out, nt_out, 5)."""
-        a = np.asarray(out_argmax, dtype=np.uint32)
-        ndm, nt = a.shape[-2], a.shape[-1]
+        """Vector version of decode_argmax over an (..., ndm_out, nt_out) token array (decoded in
+        the library). Returns an int64 array of shape (..., ndm_out, nt_out, 5)."""
+        if hasattr(out_argmax, 'detach'):
+            out_argmax = out_argmax.detach().cpu().numpy()
+        a = np.ascontiguousarray(np.asarray(out_argmax).view(np.uint32) if np.asarray(out_argmax).dtype == np.int32
+                                 else np.asarray(out_argmax, dtype=np.uint32))
+        t = self.trees[itree]
+        if a.ndim < 2 or a.shape[-2:] != (t.ndm_out, t.nt_out):
+            raise ValueError(f'decode_argmax_batch: last two axes must be (ndm_out, nt_out) = {(t.ndm_out, t.nt_out)}, got {a.shape}')
         res = np.zeros(a.shape + (5,), dtype=np.int64)
-        flat = a.reshape(-1, ndm, nt)
-        rflat = res.reshape(-1, ndm, nt, 5)
-        for b in range(flat.shape[0]):
-            for d in range(ndm):
-                for t in range(nt):
-                    rflat[b, d, t] = self.decode_argmax(int(flat[b, d, t]), itree, d, t)
+        L.check(L.lib().pdd_plan_decode_argmax_batch(self._h, itree, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), a.size,
+                                                     res.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
         return res
 
     def compute_steady_state_it0(self, itree):

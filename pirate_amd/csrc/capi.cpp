@@ -182,6 +182,22 @@ int pdd_plan_decode_argmax(const pdd_plan *p, uint32_t token, int64_t itree, int
     });
 }
 
+int pdd_plan_decode_argmax_batch(const pdd_plan *p, int64_t itree, const uint32_t *tokens, int64_t n, int64_t *out)
+{
+    PDD_TRY({
+        PDD_CHECK(p && (n == 0 || (tokens && out)), "null argument");
+        PDD_CHECK(itree >= 0 && itree < p->p->ntrees, "pdd_plan_decode_argmax_batch: itree out of range");
+        const long nd = p->p->trees[itree].ndm_out;
+        const long nt = p->p->trees[itree].nt_out;
+        PDD_CHECK(n >= 0 && n % (nd * nt) == 0, "pdd_plan_decode_argmax_batch: n must be a multiple of ndm_out * nt_out = " << nd * nt);
+        for (int64_t i = 0; i < n; i++) {
+            long v[5];
+            p->p->decode_argmax(tokens[i], itree, (i / nt) % nd, i % nt, v[0], v[1], v[2], v[3], v[4]);
+            std::copy(v, v + 5, out + 5 * i);
+        }
+    });
+}
+
 int pdd_plan_decode_argmax2(const pdd_plan *p, int64_t itree, int64_t fmin, int64_t fmax, int64_t tlo, int64_t thi,
                             int64_t pp, double *out5)
 {

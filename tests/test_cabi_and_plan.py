@@ -167,6 +167,31 @@ def test_decode_argmax_closed_forms():
         assert dm == pytest.approx(2**R * plan.config.dm_per_unit_delay(), rel=1e-12)
 
 
+def test_decode_argmax_batch_equals_scalar():
+    plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(CASES['toy']))
+    rng = np.random.default_rng(0)
+    for itree, t in enumerate(plan.trees):
+        Dout = t.nt_ds // t.nt_out
+        shape = (2, t.ndm_out, t.nt_out)
+        m = rng.integers(0, t.nmultiplets_M, size=shape)
+        p = rng.integers(0, t.nprofiles, size=shape)
+        lev = np.where(p == 0, 0, (p - 1) // 3)
+        step = np.minimum(t.Dcore, 2 ** lev)                  # token times are multiples of this
+        tt = rng.integers(0, Dout, size=shape) // step * step
+        tok = (tt | (p << 8) | (m << 16)).astype(np.uint32)
+        got = plan.decode_argmax_batch(itree, tok)
+        assert got.shape == shape + (5,)
+        for idx in [(0, 0, 0), (1, t.ndm_out - 1, t.nt_out - 1), (1, t.ndm_out // 2, t.nt_out // 3)]:
+            assert tuple(got[idx]) == plan.decode_argmax(int(tok[idx]), itree, idx[1], idx[2])
+        flat = got.reshape(-1, t.ndm_out, t.nt_out, 5)
+        for b in range(2):
+            for d in range(0, t.ndm_out, max(1, t.ndm_out // 5)):
+                for i in range(0, t.nt_out, max(1, t.nt_out // 5)):
+                    assert tuple(flat[b, d, i]) == plan.decode_argmax(int(tok[b, d, i]), itree, d, i)
+    with pytest.raises(ValueError, match='last two axes'):
+        plan.decode_argmax_batch(0, np.zeros((3, 4), np.uint32))
+
+
 def test_steady_state_it0_closed_form():
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(CASES['zones4']))
     R = plan.config.toplevel_tree_rank
