@@ -32,6 +32,11 @@ CASES = {
     'subbands': dict(BASE, zone_nfreq=[4096], zone_freq_edges=[300, 1500], toplevel_tree_rank=12,
                      time_samples_per_chunk=512, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
                      frequency_subband_counts=[4, 5, 3, 1], primary_trees=[pt(0, 32, 64, 64), pt(1, 16, 64, 64)]),
+    # Input channels much narrower than tree channels (2048 ch onto 256 tree channels, up to
+    # ~10 per tree channel): the gridding kernel stores the gridded rows for stage 1.
+    'wide_ch': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=8,
+                    time_samples_per_chunk=256, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
+                    frequency_subband_counts=[1], primary_trees=[pt(0, 8, 16, 16), pt(0, 4, 16, 16)]),
     # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
