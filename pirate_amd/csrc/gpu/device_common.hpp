@@ -61,6 +61,52 @@ __device__ __forceinline__ void st4_st(__hip_bfloat16 *p, float4 v)
     *reinterpret_cast<uint2 *>(p) = make_uint2(a | (b << 16), c | (d << 16));
 }
 
+// V consecutive storage elements <-> floats: V = 4 (16-byte float or 8-byte 16-bit access) or,
+// for 16-bit types, V = 8 (one 16-byte access).
+template<typename Tst, int V>
+__device__ __forceinline__ void ldv_st(const Tst *p, float *x)
+{
+    if constexpr (V == 4) {
+        const float4 v = ld4_st(p);
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    } else {
+        static_assert(V == 8 && sizeof(Tst) == 2, "8-wide access is for 16-bit types");
+        const uint4 u = *reinterpret_cast<const uint4 *>(p);
+        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Tst lo, hi;
+            const unsigned short l16 = (unsigned short) (w[k] & 0xffffu), h16 = (unsigned short) (w[k] >> 16);
+            __builtin_memcpy(&lo, &l16, 2);
+            __builtin_memcpy(&hi, &h16, 2);
+            x[2 * k] = ld_st(&lo);
+            x[2 * k + 1] = ld_st(&hi);
+        }
+    }
+}
+
+template<typename Tst, int V>
+__device__ __forceinline__ void stv_st(Tst *p, const float *x)
+{
+    if constexpr (V == 4) {
+        st4_st(p, make_float4(x[0], x[1], x[2], x[3]));
+    } else {
+        static_assert(V == 8 && sizeof(Tst) == 2, "8-wide access is for 16-bit types");
+        unsigned w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Tst lo, hi;
+            st_st(&lo, x[2 * k]);
+            st_st(&hi, x[2 * k + 1]);
+            unsigned short l16, h16;
+            __builtin_memcpy(&l16, &lo, 2);
+            __builtin_memcpy(&h16, &hi, 2);
+            w[k] = unsigned(l16) | (unsigned(h16) << 16);
+        }
+        *reinterpret_cast<uint4 *>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 __device__ __forceinline__ int bitrev_dev(int i, int nbits)
 {
     return (nbits > 0) ? int(__brev(unsigned(i)) >> (32 - nbits)) : 0;

@@ -66,12 +66,12 @@ __device__ __forceinline__ float quant(float x)
     }
 }
 
-// Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/4 groups of 4 consecutive
+// Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/V groups of V consecutive
 // samples at in + QS*q (q < nq; groups q >= nq are outside the chunk and left at zero). The
 // two channels' input ranges [fe, fe+ce) and [fo, fo+co) are adjacent or overlapping: each
 // channel of their union is loaded once, in batches of 4 whose loads are all issued before
 // use, and added to each row whose range contains it, in ascending channel order.
-template<typename Tst, int G, int QS>
+template<typename Tst, int G, int V, int QS>
 __device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int nq, int fe, int ce, const float *we,
                                           int fo, int co, const float *wo, float (&xe)[G], float (&xo)[G])
 {
@@ -80,13 +80,20 @@ __device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int nq, int
         xe[i] = xo[i] = 0.0f;
     const int fa = min(fe, fo), fb = max(fe + ce, fo + co);
     for (int f1 = fa; f1 < fb; f1 += 4) {
-        float4 v[4][G / 4];
+        float v[4][G];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const Tst *p = in + long(min(f1 + u, fb - 1)) * ntime;
 #pragma unroll
-            for (int q = 0; q < G / 4; q++)
-                v[u][q] = (q < nq) ? ld4_st(p + QS * q) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int q = 0; q < G / V; q++) {
+                if (q < nq) {
+                    ldv_st<Tst, V>(p + QS * q, &v[u][V * q]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < V; c++)
+                        v[u][V * q + c] = 0.0f;
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -96,22 +103,14 @@ __device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int nq, int
             if (f >= fe && f < fe + ce) {
                 const float wk = we[f - fe];
 #pragma unroll
-                for (int q = 0; q < G / 4; q++) {
-                    xe[4 * q] = xe[4 * q] + wk * v[u][q].x;
-                    xe[4 * q + 1] = xe[4 * q + 1] + wk * v[u][q].y;
-                    xe[4 * q + 2] = xe[4 * q + 2] + wk * v[u][q].z;
-                    xe[4 * q + 3] = xe[4 * q + 3] + wk * v[u][q].w;
-                }
+                for (int i = 0; i < G; i++)
+                    xe[i] = xe[i] + wk * v[u][i];
             }
             if (f >= fo && f < fo + co) {
                 const float wk = wo[f - fo];
 #pragma unroll
-                for (int q = 0; q < G / 4; q++) {
-                    xo[4 * q] = xo[4 * q] + wk * v[u][q].x;
-                    xo[4 * q + 1] = xo[4 * q + 1] + wk * v[u][q].y;
-                    xo[4 * q + 2] = xo[4 * q + 2] + wk * v[u][q].z;
-                    xo[4 * q + 3] = xo[4 * q + 3] + wk * v[u][q].w;
-                }
+                for (int i = 0; i < G; i++)
+                    xo[i] = xo[i] + wk * v[u][i];
             }
         }
     }
@@ -153,18 +152,22 @@ k_grid_lds(GridLdsArgs a)
     const int nact = min(64, ngroups - g0);
     const int t0 = g * G;
 
-    // Gridding in the coalesced layout: lane l holds samples tb0 + 256 q + 4 l + (0..3),
-    // q < G/4, so that each 16-byte load and store instruction of the wave covers 1 KB of a
-    // row. Groups past the chunk end (a partial last tile) are skipped.
+    // Gridding in the coalesced layout: lane l holds samples tb0 + 64 V q + V l + (0..V-1),
+    // q < G/V, V = 16 bytes of samples where that divides G (else 4), so that each load and
+    // store instruction of the wave covers 64 V consecutive samples of a row. With V = G this
+    // is already G consecutive samples per lane. Groups past the chunk end (a partial last
+    // tile) are skipped.
+    constexpr int V = (sizeof(Tst) == 2 && G % 8 == 0) ? 8 : 4;
+    constexpr int BLK = 64 * V;
     const int tb0 = g0 * G;
-    const int nq = min(G / 4, max(0, (a.ntime - tb0 - 4 * lane + 255) >> 8));
+    const int nq = min(G / V, max(0, (a.ntime - tb0 - V * lane + BLK - 1) / BLK));
     float xe[G], xo[G];
-    grid_pair<Tst, G, 256>(in + tb0 + 4 * lane, a.ntime, nq, fe, ce, we, fo, co, wo, xe, xo);
+    grid_pair<Tst, G, V, BLK>(in + tb0 + V * lane, a.ntime, nq, fe, ce, we, fo, co, wo, xe, xo);
 #pragma unroll
-    for (int q = 0; q < G / 4; q++) {
+    for (int q = 0; q < G / V; q++) {
         if (q < nq) {
-            st4_st(gout_e + tb0 + 256 * q + 4 * lane, make_float4(xe[4 * q], xe[4 * q + 1], xe[4 * q + 2], xe[4 * q + 3]));
-            st4_st(gout_o + tb0 + 256 * q + 4 * lane, make_float4(xo[4 * q], xo[4 * q + 1], xo[4 * q + 2], xo[4 * q + 3]));
+            stv_st<Tst, V>(gout_e + tb0 + BLK * q + V * lane, &xe[V * q]);
+            stv_st<Tst, V>(gout_o + tb0 + BLK * q + V * lane, &xo[V * q]);
         }
     }
 
@@ -196,7 +199,7 @@ k_grid_lds(GridLdsArgs a)
                     prevs[p] = rd_small[long(p - 1) * nrows + j];
             } else {
                 float xp[G], xq[G];
-                grid_pair<Tst, G, 4>(in + t0 - G, a.ntime, G / 4, fe, ce, we, fe, 0, we, xp, xq);
+                grid_pair<Tst, G, 4, 4>(in + t0 - G, a.ntime, G / 4, fe, ce, we, fe, 0, we, xp, xq);
 #pragma unroll
                 for (int i = 0; i < G; i++)
                     xp[i] = quant<Tst>(xp[i]);
@@ -228,19 +231,22 @@ k_grid_lds(GridLdsArgs a)
         __shared__ float tbuf[kGlWaves][64 * G];
         float *tb = tbuf[threadIdx.x >> 6];
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
+        for (int r = 0; r < 2 && V != G; r++) {
             float (&x)[G] = r ? xo : xe;
 #pragma unroll
-            for (int q = 0; q < G / 4; q++)
-                *reinterpret_cast<float4 *>(tb + 256 * q + 4 * lane) = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+            for (int q = 0; q < G / V; q++)
+#pragma unroll
+                for (int c = 0; c < V; c += 4)
+                    *reinterpret_cast<float4 *>(tb + BLK * q + V * lane + c) =
+                        make_float4(x[V * q + c], x[V * q + c + 1], x[V * q + c + 2], x[V * q + c + 3]);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int q = 0; q < G / 4; q++) {
-                const float4 y = *reinterpret_cast<const float4 *>(tb + G * lane + 4 * q);
-                x[4 * q] = y.x;
-                x[4 * q + 1] = y.y;
-                x[4 * q + 2] = y.z;
-                x[4 * q + 3] = y.w;
+            for (int i = 0; i < G; i += 4) {
+                const float4 y = *reinterpret_cast<const float4 *>(tb + G * lane + i);
+                x[i] = y.x;
+                x[i + 1] = y.y;
+                x[i + 2] = y.z;
+                x[i + 3] = y.w;
             }
             __builtin_amdgcn_wave_barrier();
         }
