@@ -108,7 +108,12 @@ int pdd_plan_pf_variances(const pdd_plan *plan, const double *freq_variances, in
 /* ---- GpuDedisperser (include/pirate/Dedisperser.hpp:45-392) ---- */
 
 /* GpuDedisperser::Params, Dedisperser.hpp:47-86 (stream_pool replaced by an internal
- * compute stream; cuda_device_id -> device). Zero means "default" for the counts. */
+ * compute stream; cuda_device_id -> device). Zero means "default" for the counts.
+ * enable_variances: the peak finder also writes, per output slot and tree, the float64
+ * (B, ndm_out, M, nprofiles) mean square of each (multiplet, profile)'s weighted candidates
+ * over the chunk -- the out_var of the reference peak finder (ReferenceDedisperserBase::Params::
+ * enable_variances, Dedisperser.hpp:425, 475-478; PeakFindingKernel.hpp:163-168), which the
+ * reference's GpuDedisperser does not offer. Read it with pdd_output_variances. */
 typedef struct {
     int64_t nbatches_out;
     int64_t nbatches_wt;
@@ -116,6 +121,7 @@ typedef struct {
     int32_t device;
     int32_t synchronous;
     int64_t initial_chunk;
+    int32_t enable_variances;
 } pdd_dedisperser_params;
 
 /* GpuDedisperser::create(params), src_lib/GpuDedisperser.cpp:90-372. */
@@ -145,6 +151,11 @@ int pdd_acquire_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, vo
 
 /* GpuDedisperser::release_output(consumer, seq_id, stream), GpuDedisperser.cpp:938-984. */
 int pdd_release_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream);
+
+/* out_var of an acquired, not yet released, output (requires enable_variances):
+ * out_var[itree] receives the device address of a (beams_per_batch, ndm_out, M, nprofiles)
+ * float64 array, valid until the release. No reference counterpart on the GPU. */
+int pdd_output_variances(pdd_dedisperser *dd, int64_t seq_id, double **out_var, int64_t ntrees);
 
 /* GpuDedisperser::fill_all_weights(itree, weights), GpuDedisperser.cpp:1257-1279.
  * host_weights: float32 (nbatches_wt, beams_per_batch, ndm_wt, nt_wt, nprofiles, N), >= 0. */

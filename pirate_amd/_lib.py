@@ -27,7 +27,8 @@ class TreeInfo(ctypes.Structure):
 class DedisperserParams(ctypes.Structure):
     _fields_ = [('nbatches_out', ctypes.c_int64), ('nbatches_wt', ctypes.c_int64),
                 ('num_consumers', ctypes.c_int64), ('device', ctypes.c_int32),
-                ('synchronous', ctypes.c_int32), ('initial_chunk', ctypes.c_int64)]
+                ('synchronous', ctypes.c_int32), ('initial_chunk', ctypes.c_int64),
+                ('enable_variances', ctypes.c_int32)]
 
 
 # name -> (restype, argtypes)
@@ -60,6 +61,7 @@ _SIGS = {
     'pdd_release_input_and_launch': (ctypes.c_int, [vp, i64, vp]),
     'pdd_acquire_output': (ctypes.c_int, [vp, i64, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),
     'pdd_release_output': (ctypes.c_int, [vp, i64, i64, vp]),
+    'pdd_output_variances': (ctypes.c_int, [vp, i64, ctypes.POINTER(vp), i64]),
     'pdd_dequantize': (ctypes.c_int, [ctypes.c_char_p, i64, i64, i64, vp, vp, vp, vp]),
     'pdd_ringbuf_copy_create': (ctypes.c_int, [i64, i64, i64, ctypes.POINTER(ctypes.c_uint32), i64, ctypes.POINTER(vp)]),
     'pdd_ringbuf_copy_destroy': (None, [vp]),

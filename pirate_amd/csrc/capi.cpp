@@ -60,7 +60,7 @@ static int copy_string(const std::string &s, char *buf, size_t buflen, size_t *n
 extern "C" {
 
 const char *pdd_last_error(void) { return g_last_error.c_str(); }
-int pdd_abi_version(void) { return 1; }
+int pdd_abi_version(void) { return 2; }
 
 int pdd_config_from_yaml_file(const char *filename, pdd_config **out)
 {
@@ -247,6 +247,7 @@ int pdd_dedisperser_create(const pdd_plan *p, const pdd_dedisperser_params *prm,
             gp.device = prm->device;
             gp.synchronous = prm->synchronous != 0;
             gp.initial_chunk = prm->initial_chunk;
+            gp.enable_variances = prm->enable_variances != 0;
         }
         *out = new pdd_dedisperser{std::make_unique<GpuDedisperser>(p->p, gp)};
     });
@@ -293,6 +294,17 @@ int pdd_acquire_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, vo
 int pdd_release_output(pdd_dedisperser *dd, int64_t consumer, int64_t seq_id, void *stream)
 {
     PDD_TRY_DD(dd, dd->d->release_output(consumer, seq_id, (hipStream_t) stream));
+}
+
+int pdd_output_variances(pdd_dedisperser *dd, int64_t seq_id, double **out_var, int64_t ntrees)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(out_var, "null out_var");
+        PDD_CHECK(ntrees == dd->d->ntrees, "pdd_output_variances: ntrees mismatch");
+        std::vector<double *> ov;
+        dd->d->output_variances(seq_id, ov);
+        std::copy(ov.begin(), ov.end(), out_var);
+    });
 }
 
 int pdd_fill_all_weights(pdd_dedisperser *dd, int64_t itree, const float *w, int64_t nelts)

@@ -60,7 +60,7 @@ GpuDedisperser::~GpuDedisperser()
     freeit(grid_rows);
     freeit(grid_w);
     for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &stage2_lags, &sb, &d_main_emit, &d_stag,
-                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_max, &out_argmax})
+                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_max, &out_argmax, &out_var})
         for (auto &a : *v)
             freeit(a);
     for (auto &t : timed_pending) {
@@ -348,6 +348,7 @@ void GpuDedisperser::allocate()
     weights.resize(ntrees);
     out_max.resize(ntrees);
     out_argmax.resize(ntrees);
+    out_var.resize(ntrees);
     for (long it = 0; it < ntrees; it++) {
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
@@ -374,6 +375,8 @@ void GpuDedisperser::allocate()
         weights[it] = dalloc(size_t(nbatches_wt) * B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N * sizeof(float));
         out_max[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * elt_bytes);
         out_argmax[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * sizeof(unsigned int));
+        if (params.enable_variances)
+            out_var[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * T.M * t.nprofiles * sizeof(double));
     }
 
     auto mkev = [](std::vector<hipEvent_t> &v, long n) {
@@ -522,6 +525,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         f.nt_wt = int(t.nt_wt);
         f.nt_out = int(t.nt_out);
         f.beam0 = beam0;
+        if (params.enable_variances)
+            f.out_var = (double *) out_var[it].ptr + oslot * B * t.ndm_out * T.M * t.nprofiles;
         const std::string nmp = "pf_t" + std::to_string(it);
         timed_begin(nmp, st, ev);
         launch_pf(dtype, f, st);
@@ -656,6 +661,24 @@ void GpuDedisperser::acquire_output(long consumer, long seq_id, hipStream_t stre
         const long n = beams_per_batch * t.ndm_out * t.nt_out;
         omax[it] = (char *) out_max[it].ptr + oslot * n * elt_bytes;
         oarg[it] = (unsigned int *) out_argmax[it].ptr + oslot * n;
+    }
+}
+
+
+void GpuDedisperser::output_variances(long seq_id, std::vector<double *> &ov)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(params.enable_variances, "output_variances(): the dedisperser was created without enable_variances");
+    bool held = false;
+    for (long c = 0; c < num_consumers; c++)
+        held = held || (seq_id >= curr_output_release[c] && seq_id < curr_output_acquire[c]);
+    PDD_CHECK(held, "output_variances(): seq_id " << seq_id << " is not an acquired output");
+    const long oslot = seq_id % nbatches_out;
+    ov.resize(ntrees);
+    for (long it = 0; it < ntrees; it++) {
+        const DedispersionTree &t = plan->trees[it];
+        ov[it] = (double *) out_var[it].ptr + oslot * beams_per_batch * t.ndm_out * tables[it].M * t.nprofiles;
     }
 }
 

@@ -29,6 +29,7 @@ struct GpuDedisperserParams {
     int device = 0;
     bool synchronous = false;
     long initial_chunk = 0;
+    bool enable_variances = false;   // peak finder also writes out_var (float64) per output slot
 };
 
 struct DeviceArray {
@@ -63,6 +64,10 @@ public:
     void acquire_output(long consumer, long seq_id, hipStream_t stream, bool sync,
                         std::vector<void *> &out_max, std::vector<unsigned int *> &out_argmax);
     void release_output(long consumer, long seq_id, hipStream_t stream);
+
+    // out_var[itree]: device address of the (B, ndm_out, M, nprofiles) float64 array of an
+    // acquired (not yet released) seq_id; requires enable_variances.
+    void output_variances(long seq_id, std::vector<double *> &out_var);
 
     // host: (nbatches_wt, B, ndm_wt, nt_wt, P, N) float32, all entries finite and >= 0.
     void fill_all_weights(long itree, const float *host, long nelts);
@@ -123,6 +128,7 @@ private:
     std::vector<DeviceArray> s2_state; std::vector<long> s2_state_bstride;
     std::vector<DeviceArray> weights; // [itree] (nbatches_wt, B, ndm_wt, nt_wt, P, N) float
     std::vector<DeviceArray> out_max, out_argmax;   // [itree] (nbatches_out, B, ndm_out, nt_out)
+    std::vector<DeviceArray> out_var; // [itree] (nbatches_out, B, ndm_out, M, P) float64, if enabled
     DeviceArray grid_rows;            // int (3*nchan) f0, cnt, woff
     DeviceArray grid_w;               // float
 

@@ -63,12 +63,35 @@ struct PfShape {
     static constexpr int WAVES = (WTOT + P <= 32 && LD >= 2) ? 4 : 2;
 };
 
-template<typename S>
+template<typename S, bool VAR>
 struct PfLane {
     float w[S::WTOT];          // per level l: w[woff(l) + i] = B_l(e - i*dt(l)), e = newest end
     float bv[S::NLEV];         // per level: best weighted value in the current bin
     unsigned int bc[S::NLEV];  // per level: its code (see code())
     float wt[S::P];            // weights of the current bin, per profile
+    // VAR: per profile, the sum of squared weighted candidates in the current bin (bs) and
+    // the chunk's running sum of bs / (nt_out * candidates per bin at that level) (acc),
+    // in float64 -- the reference peak finder's optional out_var.
+    double bs[VAR ? S::P : 1], acc[VAR ? S::P : 1], wv[S::NLEV];
+
+    template<int P>
+    __device__ __forceinline__ void sq(float v)
+    {
+        if constexpr (VAR)
+            bs[P] = bs[P] + double(v) * double(v);
+    }
+
+    // End of an output bin: fold the bin's sums into the chunk's (same order as the reference).
+    __device__ __forceinline__ void fold_var()
+    {
+        if constexpr (VAR) {
+#pragma unroll
+            for (int p = 0; p < S::P; p++) {
+                acc[p] = acc[p] + bs[p] * wv[(p == 0) ? 0 : (p - 1) / 3];
+                bs[p] = 0.0;
+            }
+        }
+    }
 
     // Candidate codes within a level. Compact codes isamp*4 + q (q = 0 for p = 0, else
     // p - 3l) stay within the hardware's inline-constant range (<= 64) when Dout <= 16, so the
@@ -124,17 +147,24 @@ struct PfLane {
         constexpr int isamp = (POS + 1) / S::dt(L) - 1;
         constexpr bool first = (isamp == 0);
         const float x3 = w[o], x2 = w[o + s], x1 = w[o + 2 * s], x0 = w[o + 3 * s];
-        if constexpr (L == 0)
-            upd<0, first>(wt[0] * x3, code<0>(isamp, 0));
+        if constexpr (L == 0) {
+            const float v0 = wt[0] * x3;
+            upd<0, first>(v0, code<0>(isamp, 0));
+            sq<0>(v0);
+        }
         if constexpr (S::P > 1) {
             constexpr int p = 3 * L + 1;
             const float h3 = 0.5f * x3;
             const float y1 = x2 + x3;
             const float y2 = (0.5f * x1 + x2) + h3;
             const float y3 = ((0.5f * x0 + x1) + x2) + h3;
-            upd<L, first && (L > 0)>(wt[p] * y1, code<L>(isamp, 1));
-            upd<L, false>(wt[p + 1] * y2, code<L>(isamp, 2));
-            upd<L, false>(wt[p + 2] * y3, code<L>(isamp, 3));
+            const float v1 = wt[p] * y1, v2 = wt[p + 1] * y2, v3 = wt[p + 2] * y3;
+            upd<L, first && (L > 0)>(v1, code<L>(isamp, 1));
+            upd<L, false>(v2, code<L>(isamp, 2));
+            upd<L, false>(v3, code<L>(isamp, 3));
+            sq<p>(v1);
+            sq<p + 1>(v2);
+            sq<p + 2>(v3);
         }
     }
 
@@ -191,8 +221,8 @@ __device__ __forceinline__ int pf_level_of_token(unsigned int tok)
     return (p == 0) ? 0 : (p - 1) / 3;
 }
 
-template<typename Tst, int LW, int LD>
-__global__ void __launch_bounds__(kPfThreads) __attribute__((amdgpu_waves_per_eu(PfShape<LW, LD>::WAVES)))
+template<typename Tst, int LW, int LD, bool VAR>
+__global__ void __launch_bounds__(kPfThreads) __attribute__((amdgpu_waves_per_eu(VAR ? 2 : PfShape<LW, LD>::WAVES)))
 k_pf(PfArgs a)
 {
     using S = PfShape<LW, LD>;
@@ -219,10 +249,18 @@ k_pf(PfArgs a)
     Tst *outm = reinterpret_cast<Tst *>(a.out_max) + long(b) * a.out_bstride;
     unsigned int *outa = a.out_argmax + long(b) * a.out_bstride;
 
-    PfLane<S> ln;
+    PfLane<S, VAR> ln;
 #pragma unroll
     for (int i = 0; i < S::WTOT; i++)
         ln.w[i] = 0.0f;
+    if constexpr (VAR) {
+#pragma unroll
+        for (int p = 0; p < S::P; p++)
+            ln.bs[p] = ln.acc[p] = 0.0;
+#pragma unroll
+        for (int l = 0; l < S::NLEV; l++)
+            ln.wv[l] = 1.0 / double(a.nt_out * (S::DOUT / S::dt(l)));
+    }
 
     const int nwarm = (a.tpad + TL - 1) / TL;
     const int ntiles = nwarm + a.Tp / TL;
@@ -265,6 +303,7 @@ k_pf(PfArgs a)
                     for (int p = 0; p < S::P; p++)
                         ln.wt[p] = wp[p * N];
                     ln.template bin<true>(xs + j * S::DOUT);
+                    ln.fold_var();
                     float V;
                     unsigned int C;
                     ln.combine(V, C);
@@ -301,6 +340,15 @@ k_pf(PfArgs a)
         }
     }
 
+    if constexpr (VAR) {
+        if (active) {
+            double *ov = a.out_var + long(b) * a.ndm_out * M * S::P + (long(dpf0 + dl) * M + m) * S::P;
+#pragma unroll
+            for (int p = 0; p < S::P; p++)
+                ov[p] = ln.acc[p];
+        }
+    }
+
     // ---- rotate rows for the next chunk: positions [Tp, Tp + tpad + flmax) -> [0, ...)
     // (all staging loads completed before the last barrier; Tp >= tpad + flmax is checked
     // on the host, so no position is both read and written here)
@@ -318,9 +366,12 @@ static void launch_pf_lw(const PfArgs &a, hipStream_t st, dim3 grid)
 {
     dim3 block(kPfThreads);
     switch (a.Dout) {
-#define PDD_PF(LDV)                                                          \
-        case (1 << LDV):                                                     \
-            hipLaunchKernelGGL((k_pf<Tst, LW, LDV>), grid, block, 0, st, a); \
+#define PDD_PF(LDV)                                                                \
+        case (1 << LDV):                                                           \
+            if (a.out_var)                                                         \
+                hipLaunchKernelGGL((k_pf<Tst, LW, LDV, true>), grid, block, 0, st, a);  \
+            else                                                                   \
+                hipLaunchKernelGGL((k_pf<Tst, LW, LDV, false>), grid, block, 0, st, a); \
             break;
         PDD_PF(0) PDD_PF(1) PDD_PF(2) PDD_PF(3) PDD_PF(4) PDD_PF(5)
 #undef PDD_PF

@@ -114,6 +114,8 @@ struct PfArgs {
     long out_bstride;
     int B, ndm_out, M, N, P, nlev, Tp, Dout, Dcore, ndm_wt, nt_wt, nt_out;
     int beam0;                // global index of the batch's first beam (sb rows are per global beam)
+    double *out_var = nullptr;   // optional (B, ndm_out, M, P) float64: per-chunk mean square of each
+                                 // (multiplet, profile)'s weighted candidates
 };
 bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp);
 void launch_pf(Dtype dt, const PfArgs &a, hipStream_t st);

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib as L
 
-_TYPESTR = {'float32': '<f4', 'float16': '<f2', 'bfloat16': '<V2'}
+_TYPESTR = {'float32': '<f4', 'float16': '<f2', 'bfloat16': '<V2', 'float64': '<f8'}
 
 
 class _DeviceView:
@@ -66,9 +66,10 @@ class GpuDedisperserOutputs:
     """Outputs of one beam batch: out_max[itree], out_argmax[itree] (tensor views valid until
     the matching release). Reference: pirate_frb/core/GpuDedisperserOutputs.py."""
 
-    def __init__(self, out_max, out_argmax, ichunk_zero_based, ichunk_fpga_based, ibeam):
+    def __init__(self, out_max, out_argmax, ichunk_zero_based, ichunk_fpga_based, ibeam, out_var=None):
         self.out_max = out_max
         self.out_argmax = out_argmax
+        self.out_var = out_var      # [itree] float64 (B, ndm_out, M, nprofiles) if enable_variances
         self.ichunk_zero_based = ichunk_zero_based
         self.ichunk_fpga_based = ichunk_fpga_based
         self.ibeam = ibeam
@@ -90,7 +91,7 @@ class GpuDedisperser:
     """
 
     def __init__(self, plan, stream_pool=None, cuda_device_id=0, num_consumers=1, nbatches_out=0, nbatches_wt=0,
-                 initial_chunk=0, synchronous=False):
+                 initial_chunk=0, synchronous=False, enable_variances=False):
         self.plan = plan
         self.config = plan.config
         self.dtype = plan.dtype
@@ -107,8 +108,9 @@ class GpuDedisperser:
         self.nbatches_wt = nbatches_wt if nbatches_wt > 0 else self.nbatches_out
         self.initial_chunk = initial_chunk
         self.cuda_device_id = cuda_device_id
+        self.enable_variances = bool(enable_variances)
         p = L.DedisperserParams(nbatches_out, nbatches_wt, self.num_consumers, cuda_device_id, int(synchronous),
-                                initial_chunk)
+                                initial_chunk, int(self.enable_variances))
         self._h = ctypes.c_void_p()
         L.check(L.lib().pdd_dedisperser_create(plan._h, ctypes.byref(p), ctypes.byref(self._h)))
 
@@ -151,9 +153,15 @@ class GpuDedisperser:
             shape = (self.beams_per_batch, t.ndm_out, t.nt_out)
             mx.append(device_view(om[i], shape, self.dtype))
             am.append(device_view(oa[i], shape, 'uint32'))
+        ov = None
+        if self.enable_variances:
+            vp = (ctypes.c_void_p * n)()
+            L.check(L.lib().pdd_output_variances(self._h, seq_id, vp, n))
+            ov = [device_view(vp[i], (self.beams_per_batch, t.ndm_out, t.nmultiplets_M, t.nprofiles), 'float64')
+                  for i, t in enumerate(self.trees)]
         ichunk = seq_id // self.nbatches
         return GpuDedisperserOutputs(mx, am, ichunk, ichunk + self.initial_chunk,
-                                     (seq_id % self.nbatches) * self.beams_per_batch)
+                                     (seq_id % self.nbatches) * self.beams_per_batch, ov)
 
     def _release_output(self, consumer_id, seq_id, stream_ptr):
         L.check(L.lib().pdd_release_output(self._h, consumer_id, seq_id, ctypes.c_void_p(stream_ptr)))

@@ -16,16 +16,17 @@ def random_weights(rng, dd, itree):
     return w.astype(np.float32)
 
 
-def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print):
-    """Returns a list of per-(seq, tree) mismatch records (empty if bit-exact)."""
+def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, variances=False):
+    """Returns a list of per-(seq, tree) mismatch records (empty if bit-exact). With
+    variances=True the peak finder's float64 out_var is compared too (exactly)."""
     import torch
     import pirate_amd as pa
 
     cfg = pa.DedispersionConfig.from_dict(cfg_dict)
     plan = pa.DedispersionPlan(cfg)
-    dd = pa.GpuDedisperser(plan)
+    dd = pa.GpuDedisperser(plan, enable_variances=variances)
     dd.allocate()
-    ref = orc.Dedisperser(oracle_config(cfg_dict), 1)
+    ref = orc.Dedisperser(oracle_config(cfg_dict), 1, enable_variances=variances)
     rng = np.random.default_rng(seed)
 
     W = [random_weights(rng, dd, it) for it in range(plan.ntrees)]
@@ -53,6 +54,15 @@ def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print):
                     rm, ra = ref.output(it)
                     nm = int(np.sum(gm != rm))
                     na = int(np.sum(ga != ra))
+                    if variances:
+                        gv = out.out_var[it].cpu().numpy()
+                        rv = ref.variances(it)
+                        nv = int(np.sum(gv != rv))
+                        if nv:
+                            rec = dict(seq=seq, itree=it, n_var_mismatch=nv, total=int(gv.size),
+                                       max_rel=float(np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1e-300))))
+                            bad.append(rec)
+                            log(f'MISMATCH {rec}')
                     if nm or na:
                         ev = ref.eval_tokens(it, ga)
                         rec = dict(seq=seq, itree=it, n_max_mismatch=nm, n_argmax_mismatch=na,
