@@ -36,6 +36,28 @@ def test_library_exports_every_declared_symbol():
     assert L.pdd_abi_version() == 2
 
 
+def test_plain_c_caller(tmp_path):
+    """The header compiles as C (gcc -std=c99) and a C program linked against the library
+    drives the host-side entry points (tests/c/capi_host.c)."""
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip('gcc not available')
+    _lib.lib()                                  # the library is built
+    exe = str(tmp_path / 'capi_host')
+    libdir = os.path.dirname(_lib.LIBPATH)
+    subprocess.run(['gcc', '-std=c99', '-Wall', '-Werror', '-I', os.path.join(ROOT, 'include'),
+                    os.path.join(HERE, 'c', 'capi_host.c'), '-o', exe, '-L', libdir, '-lpirate_amd',
+                    '-Wl,-rpath,' + libdir], check=True, capture_output=True, text=True)
+    env = dict(os.environ)
+    torch_lib = os.path.join(os.path.dirname(__import__('torch').__file__), 'lib')
+    env['LD_LIBRARY_PATH'] = ':'.join(x for x in (torch_lib, '/opt/rocm/lib', env.get('LD_LIBRARY_PATH', '')) if x)
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count('ok ') == 9
+
+
 def golden():
     with open(os.path.join(HERE, 'golden', 'reference_plans.json')) as f:
         return json.load(f)
