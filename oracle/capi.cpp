@@ -230,14 +230,16 @@ extern "C" int orc_pf_eval_tokens(void *hv, void *out, const uint32_t *tokens, c
     });
 }
 
-extern "C" void *orc_dd_create(const orc_cfg *c, int soph, int dbl, int enable_var, int tree_domain_input)
+extern "C" void *orc_dd_create(const orc_cfg *c, int soph, int dbl, int enable_var, int tree_domain_input, int storage)
 {
     try {
         OrcConfig cfg = to_cfg(c);
+        if (storage != 0 && dbl)
+            throw std::runtime_error("orc_dd_create: 16-bit storage rounding needs the float oracle");
         auto *h = new Handle;
         h->kind = 5; h->dbl = dbl;
         if (dbl) h->d = std::make_shared<RefDedisperser<double>>(cfg, soph, enable_var != 0, tree_domain_input != 0);
-        else h->f = std::make_shared<RefDedisperser<float>>(cfg, soph, enable_var != 0, tree_domain_input != 0);
+        else h->f = std::make_shared<RefDedisperser<float>>(cfg, soph, enable_var != 0, tree_domain_input != 0, storage);
         return h;
     }
     catch (std::exception &e) { g_err = e.what(); return nullptr; }
@@ -341,5 +343,11 @@ extern "C" int orc_ringbuf_copy(uint8_t *ringbuf, long nbytes, const uint32_t *o
             }
         }
     ref_ringbuf_copy(ringbuf, oct, noct, seg_bytes, nbeams, iframe);
+    return 0;
+}
+
+extern "C" int orc_round_storage(float *x, long n, int storage)
+{
+    round_to_storage(x, n, storage);
     return 0;
 }

@@ -31,6 +31,8 @@
 #include <string>
 #include <vector>
 
+inline float half_bits_to_float(uint16_t h);   // IEEE binary16 -> float (defined below)
+
 namespace oracle {
 
 #define ORC_CHECK(cond, msg)                                                  \
@@ -704,6 +706,64 @@ struct OrcConfig {
 
 
 // -------------------------------------------------------------------------------------------
+// Storage rounding of the 16-bit tiers: x rounded to IEEE binary16 or bfloat16 with
+// round-to-nearest-even, as a store to the storage type and a reload would give.
+
+inline uint16_t float_to_half_bits(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    x &= 0x7fffffffu;
+    if (x >= 0x7f800000u)                                  // inf, nan (quiet)
+        return uint16_t(sign | 0x7c00u | ((x > 0x7f800000u) ? 0x200u : 0u));
+    if (x >= 0x477ff000u)                                  // >= 65520: rounds to inf
+        return uint16_t(sign | 0x7c00u);
+    if (x >= 0x38800000u) {                                // normal binary16 (>= 2^-14)
+        uint32_t h = ((((x >> 23) - 127 + 15)) << 10) | ((x & 0x7fffffu) >> 13);
+        const uint32_t rem = x & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (h & 1u)))
+            h++;                                           // a carry into the exponent is exact
+        return uint16_t(sign | h);
+    }
+    // subnormal binary16: units of 2^-24, nearest-even integer (exact in double)
+    float a;
+    std::memcpy(&a, &x, 4);
+    const double u = double(a) * 16777216.0;
+    double r = std::floor(u);
+    const double d = u - r;
+    if (d > 0.5 || (d == 0.5 && std::fmod(r, 2.0) != 0.0))
+        r += 1.0;
+    return uint16_t(sign | uint32_t(r));                   // r = 1024 is the smallest normal
+}
+
+inline float round_to_half(float x) { return ::half_bits_to_float(float_to_half_bits(x)); }
+
+inline float round_to_bf16(float x)
+{
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    if ((b & 0x7fffffffu) > 0x7f800000u)
+        b = (b | 0x00400000u) & 0xffff0000u;               // nan stays a (quiet) nan
+    else
+        b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
+
+// storage: 0 = float32 (no rounding), 1 = float16, 2 = bfloat16.
+template<typename T>
+inline void round_to_storage(T *x, long n, int storage)
+{
+    if (storage == 1)
+        for (long i = 0; i < n; i++) x[i] = T(round_to_half(float(x[i])));
+    else if (storage == 2)
+        for (long i = 0; i < n; i++) x[i] = T(round_to_bf16(float(x[i])));
+}
+
+
+// -------------------------------------------------------------------------------------------
 // Whole dedisperser, sophistication 0 and 1.
 // Reference: src_lib/ReferenceDedisperser.cpp:48-450, include/pirate/Dedisperser.hpp:395-482.
 
@@ -712,6 +772,7 @@ struct RefDedisperser {
     OrcConfig cfg;
     int soph;
     bool tree_domain_input;
+    int storage = 0;   // soph 1: round to 16-bit storage where the GPU chain stores (see dedisperse())
     std::vector<OrcTree> trees;
     long B, nbatches, ntrees, npri, R, nt_in, nfreq;
     std::vector<double> cmap;
@@ -738,10 +799,11 @@ struct RefDedisperser {
     std::vector<RefLagbuf<T>> s2_seg_lagbufs;            // (nbatches * ntrees)
     std::vector<RefLagbuf<T>> s2_res_lagbufs;            // (nbatches * ntrees)
 
-    RefDedisperser(const OrcConfig &c, int soph_, bool enable_var, bool tdi = false) :
-        cfg(c), soph(soph_), tree_domain_input(tdi)
+    RefDedisperser(const OrcConfig &c, int soph_, bool enable_var, bool tdi = false, int storage_ = 0) :
+        cfg(c), soph(soph_), tree_domain_input(tdi), storage(storage_)
     {
         ORC_CHECK(soph == 0 || soph == 1, "sophistication must be 0 or 1");
+        ORC_CHECK(storage == 0 || soph == 1, "16-bit storage rounding is modelled for sophistication 1 only");
         trees = cfg.make_trees();
         B = cfg.beams_per_batch;
         nbatches = cfg.beams_per_gpu / B;
@@ -870,18 +932,24 @@ struct RefDedisperser {
             return;
         }
 
-        // soph 1
+        // soph 1. With 16-bit storage, values are rounded where the GPU chain stores them:
+        // gridded rows, lagged-downsampling outputs, stage-1 outputs (the lag ring), stage-2
+        // subband rows (the peak-finder input) and out_max; everything in between is float.
         grid_or_copy(s1_bufs[0].data());
+        round_to_storage(s1_bufs[0].data(), long(s1_bufs[0].size()), storage);
         {
             std::vector<T *> outs;
             for (long p = 1; p < npri; p++) outs.push_back(s1_bufs[p].data());
             lds[ibatch]->apply(s1_bufs[0].data(), outs.data());
+            for (long p = 1; p < npri; p++)
+                round_to_storage(s1_bufs[p].data(), long(s1_bufs[p].size()), storage);
         }
         for (long p = 0; p < npri; p++) {
             long rp = p ? R - 1 : R;
             long dd = rp / 2, amb = rp - dd, nt = nt_in / pow2(p);
             T *buf = s1_bufs[p].data();
             s1_trees[ibatch * npri + p]->dedisperse(buf, pow2(rp) * nt, pow2(dd) * nt, nt, nullptr, 0, 0, 0);
+            round_to_storage(buf, long(s1_bufs[p].size()), storage);
             (void) amb;
         }
         for (long it = 0; it < ntrees; it++) {
@@ -907,7 +975,9 @@ struct RefDedisperser {
             std::vector<T> sb(B * t.ndm_out * t.M * nt);
             s2_trees[ibatch * ntrees + it]->dedisperse(dd, pow2(rank) * nt, nt, A * nt,
                                                       sb.data(), t.ndm_out * t.M * nt, t.M * nt, nt);
+            round_to_storage(sb.data(), long(sb.size()), storage);
             run_pf(it, sb.data(), ibatch);
+            round_to_storage(out_max[it].data(), long(out_max[it].size()), storage);
         }
     }
 };

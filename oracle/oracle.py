@@ -133,6 +133,16 @@ class OracleConfig:
         return ret
 
 
+_STORAGE = {'float32': 0, 'float16': 1, 'bfloat16': 2}
+
+
+def round_to_storage(x, storage):
+    """float32 array rounded to 'float16' / 'bfloat16' (round-to-nearest-even), as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32).copy()
+    _check(lib().orc_round_storage(_ptr(a), ctypes.c_long(a.size), _STORAGE[storage]))
+    return a
+
+
 def _dt(dbl):
     return np.float64 if dbl else np.float32
 
@@ -281,7 +291,10 @@ class PeakFinder(_Obj):
 class Dedisperser(_Obj):
     """Reference dedisperser (ReferenceDedisperser), sophistication 0 or 1."""
 
-    def __init__(self, cfg, sophistication, dbl=False, enable_variances=False, tree_domain_input=False):
+    def __init__(self, cfg, sophistication, dbl=False, enable_variances=False, tree_domain_input=False,
+                 storage='float32'):
+        """storage 'float16' / 'bfloat16' (float oracle, sophistication 1): values are rounded to
+        the storage type where the GPU chain stores them (oracle.hpp, RefDedisperser::dedisperse)."""
         self.cfg, self.soph, self.dbl = cfg, sophistication, dbl
         self.trees = cfg.trees()
         self.B = cfg.beams_per_batch
@@ -290,7 +303,7 @@ class Dedisperser(_Obj):
         self.enable_variances = enable_variances
         c = cfg._c()
         self._h = lib().orc_dd_create(ctypes.byref(c), int(sophistication), int(dbl), int(enable_variances),
-                                      int(tree_domain_input))
+                                      int(tree_domain_input), _STORAGE[storage])
         if not self._h:
             raise RuntimeError(lib().orc_last_error().decode())
 

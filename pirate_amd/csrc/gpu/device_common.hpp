@@ -20,8 +20,18 @@ __device__ __forceinline__ float ld_st(const float *p) { return *p; }
 __device__ __forceinline__ float ld_st(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ float ld_st(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
 
+// The value as computed in float32. Without this the compiler may fuse the multiply that
+// produced x into the float16 conversion (v_fma_mix*_f16), rounding once from the exact
+// product to float16 instead of rounding to float32 and then to float16 -- a different
+// value in rare cases. The empty constraint emits no instruction.
+__device__ __forceinline__ float as_f32(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ void st_st(float *p, float x) { *p = x; }
-__device__ __forceinline__ void st_st(__half *p, float x) { *p = __float2half(x); }
+__device__ __forceinline__ void st_st(__half *p, float x) { *p = __float2half(as_f32(x)); }
 __device__ __forceinline__ void st_st(__hip_bfloat16 *p, float x) { *p = __float2bfloat16(x); }
 
 // Four consecutive storage elements (16-byte aligned for float, 8-byte for 16-bit types).
@@ -49,8 +59,8 @@ __device__ __forceinline__ void st4_st(float *p, float4 v) { *reinterpret_cast<f
 
 __device__ __forceinline__ void st4_st(__half *p, float4 v)
 {
-    const unsigned a = __half_as_ushort(__float2half(v.x)), b = __half_as_ushort(__float2half(v.y));
-    const unsigned c = __half_as_ushort(__float2half(v.z)), d = __half_as_ushort(__float2half(v.w));
+    const unsigned a = __half_as_ushort(__float2half(as_f32(v.x))), b = __half_as_ushort(__float2half(as_f32(v.y)));
+    const unsigned c = __half_as_ushort(__float2half(as_f32(v.z))), d = __half_as_ushort(__float2half(as_f32(v.w)));
     *reinterpret_cast<uint2 *>(p) = make_uint2(a | (b << 16), c | (d << 16));
 }
 

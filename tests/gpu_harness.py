@@ -18,7 +18,8 @@ def random_weights(rng, dd, itree):
 
 def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, variances=False):
     """Returns a list of per-(seq, tree) mismatch records (empty if bit-exact). With
-    variances=True the peak finder's float64 out_var is compared too (exactly)."""
+    variances=True the peak finder's float64 out_var is compared too (exactly). With a 16-bit
+    dtype in cfg_dict, the oracle rounds to that type where the GPU chain stores it."""
     import torch
     import pirate_amd as pa
 
@@ -26,7 +27,8 @@ def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, 
     plan = pa.DedispersionPlan(cfg)
     dd = pa.GpuDedisperser(plan, enable_variances=variances)
     dd.allocate()
-    ref = orc.Dedisperser(oracle_config(cfg_dict), 1, enable_variances=variances)
+    storage = cfg_dict['dtype']
+    ref = orc.Dedisperser(oracle_config(cfg_dict), 1, enable_variances=variances, storage=storage)
     rng = np.random.default_rng(seed)
 
     W = [random_weights(rng, dd, it) for it in range(plan.ntrees)]
@@ -42,6 +44,8 @@ def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, 
                 x = rng.uniform(-1, 1, size=(B, F, T)).astype(np.float32)
             else:
                 x = rng.standard_normal((B, F, T)).astype(np.float32)
+            if storage != 'float32':
+                x = orc.round_to_storage(x, storage).reshape(x.shape)
             with dd.get_input(seq) as arr:
                 arr.copy_(torch.from_numpy(x).to(arr.device))
             for it in range(plan.ntrees):
@@ -49,7 +53,7 @@ def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, 
             ref.dedisperse(x, ichunk, ib)
             with dd.get_output(seq) as out:
                 for it in range(plan.ntrees):
-                    gm = out.out_max[it].cpu().numpy()
+                    gm = out.out_max[it].float().cpu().numpy()
                     ga = out.out_argmax[it].cpu().numpy().view(np.uint32)
                     rm, ra = ref.output(it)
                     nm = int(np.sum(gm != rm))

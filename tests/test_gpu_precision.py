@@ -29,3 +29,15 @@ def test_reduced_precision_vs_fp64_oracle(name, dtype):
         assert s['max'] <= tol['max'], (it, s)
         assert s['rms'] <= tol['rms'], (it, s)
         assert s['tok_max'] <= tol['max'], (it, s)
+
+
+@pytest.mark.parametrize('dtype', ['float16', 'bfloat16'])
+@pytest.mark.parametrize('name', ['toy', 'zones4', 'subbands', 'cfg1'])
+def test_16bit_storage_bit_exact_vs_rounding_oracle(name, dtype):
+    """With 16-bit storage the GPU rounds to the storage type only where it stores (gridded
+    rows, downsampled rows, lag ring, subband rows, out_max) and computes in float32 in
+    between; the oracle rounding at the same points gives identical out_max and argmax."""
+    from gpu_cases import CASES
+    from gpu_harness import run_and_compare
+    bad, dd, plan = run_and_compare(dict(CASES[name], dtype=dtype), 3, seed=23, log=print)
+    assert not bad, f'{len(bad)} mismatching (seq, tree) outputs, first: {bad[0]}'

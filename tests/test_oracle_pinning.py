@@ -414,3 +414,52 @@ def test_oracle_trees_match_reference_plan(orc):
         assert t['ndm_out'] == g['ndm_out'] and t['nt_out'] == g['nt_out']
         assert t['ndm_wt'] == g['ndm_wt'] and t['nt_wt'] == g['nt_wt']
         assert t['subband_counts'] == g['frequency_subband_counts']
+
+
+# ----------------------------------------------------------------------------------------------
+# 16-bit storage rounding (the oracle's model of the GPU chain's fp16/bf16 stores)
+
+@pytest.mark.parametrize('storage', ['float16', 'bfloat16'])
+def test_storage_rounding_matches_library_casts(storage, orc):
+    """Round-to-nearest-even to binary16 / bfloat16 equals numpy's float16 cast and torch's
+    bfloat16 cast on normal, subnormal, overflowing, tie and special values."""
+    import torch
+    rng = np.random.default_rng(0)
+    parts = [rng.standard_normal(20000).astype(np.float32) * s for s in (1e-9, 6e-8, 1e-5, 1.0, 1e3, 6e4, 7e4, 1e30)]
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 65504, 65519.99, 65520, 65536, 2.0**-24, 2.0**-25,
+                        3 * 2.0**-26, 2.0**-14, 2.0**-14 * (1 - 2.0**-11), 1 + 2.0**-11, 1 + 3 * 2.0**-11,
+                        1 + 2.0**-8, 1 + 3 * 2.0**-8], dtype=np.float32)
+    x = np.concatenate(parts + [special])
+    got = orc.round_to_storage(x, storage)
+    with np.errstate(over='ignore'):
+        want = (x.astype(np.float16).astype(np.float32) if storage == 'float16'
+                else torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy())
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    num = ~np.isnan(want)
+    assert np.all(same) and np.array_equal(np.signbit(got[num]), np.signbit(want[num]))   # a nan's sign is free
+
+
+def test_storage_rounding_oracle_is_close_to_float(orc):
+    """The 16-bit-storage oracle's outputs are representable in the storage type and within
+    the storage precision of the float oracle's."""
+    from gpu_cases import CASES
+    d = dict(CASES['toy'])
+    cfg = orc.OracleConfig.from_yaml_dict(d, nelts_per_segment=64)
+    a = orc.Dedisperser(cfg, 1)
+    b = orc.Dedisperser(cfg, 1, storage='float16')
+    rng = np.random.default_rng(3)
+    w = [orc.bare_weights(*a.wt_shape(it)) for it in range(len(a.trees))]
+    for it in range(len(a.trees)):
+        a.set_weights(it, w[it])
+        b.set_weights(it, w[it])
+    for c in range(3):
+        for ib in range(a.nbatches):
+            x = orc.round_to_storage(rng.uniform(-1, 1, size=(a.B, cfg.nfreq, cfg.nt_in)), 'float16')
+            x = x.reshape(a.B, cfg.nfreq, cfg.nt_in)
+            a.dedisperse(x, c, ib)
+            b.dedisperse(x, c, ib)
+            for it in range(len(a.trees)):
+                ma, _ = a.output(it)
+                mb, _ = b.output(it)
+                assert np.array_equal(mb, orc.round_to_storage(mb, 'float16').reshape(mb.shape))
+                assert np.max(np.abs(ma - mb)) <= 0.02 * max(1.0, float(np.max(np.abs(ma))))
