@@ -314,14 +314,25 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
             }
         }
     };
+    // Stage 1 (and stage 2 below rank 7) issues the next tile's phase-A loads right after
+    // this tile's phase A, so they are in flight during phase B; for rank-7 stage 2 the extra
+    // registers would spill, and measured slower (1.31 vs 1.19 ms on chord16k).
+    constexpr bool PREFETCH = !STAGE2 || DD <= 6;
+    if constexpr (PREFETCH)
+        load_a(0);
     for (int t0 = 0; t0 < a.Tp; t0 += 64) {
-        load_a(t0);
+        if constexpr (!PREFETCH)
+            load_a(t0);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             sub_passes<F::K1>(xa[s], ha[s], lane);
             phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
         }
         __syncthreads();
+        if constexpr (PREFETCH) {
+            if (t0 + 64 < a.Tp)
+                load_a(t0 + 64);
+        }
 
 #pragma unroll
         for (int s = 0; s < F::SB; s++) {
