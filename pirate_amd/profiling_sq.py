@@ -1,9 +1,14 @@
 """Per-kernel SQ counter summary of a rocprofv3 --pmc run of bench.py (chain names as in
-pirate_amd.profiling). Usage: python -m pirate_amd.profiling_sq DIR [--npri 4 --ntrees 4]"""
+pirate_amd.profiling). Usage: python -m pirate_amd.profiling_sq DIR [--npri 4 --ntrees 4]
+
+Prints the average of every counter per chain kernel and, when the counters are present,
+instructions per wave (SQ_INSTS_* / SQ_WAVES) and the share of wave cycles spent waiting
+on any instruction (SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES)."""
 
 import argparse
 import collections
 import csv
+import glob
 import os
 import sys
 
@@ -16,7 +21,10 @@ def main(argv=None):
     ap.add_argument('--npri', type=int, default=4)
     ap.add_argument('--ntrees', type=int, default=4)
     a = ap.parse_args(argv)
-    rows = list(csv.DictReader(open(os.path.join(a.dir, 'run_counter_collection.csv'))))
+    paths = glob.glob(os.path.join(a.dir, '**', '*counter_collection.csv'), recursive=True)
+    if not paths:
+        raise SystemExit(f'no *counter_collection.csv under {a.dir}')
+    rows = list(csv.DictReader(open(paths[0])))
     per = collections.defaultdict(dict)
     kn = {}
     for r in rows:
@@ -31,11 +39,24 @@ def main(argv=None):
             for c, v in per[d].items():
                 acc[lab][c].append(v)
     names = sorted({c for d in per.values() for c in d})
-    print('| kernel | ' + ' | '.join(names) + ' |')
-    print('|---|' + '---|' * len(names))
+    derived = []
+    if 'SQ_WAVES' in names:
+        derived += [(c.replace('SQ_INSTS_', '') + '/wave', c) for c in names if c.startswith('SQ_INSTS_')]
+    if 'SQ_WAIT_INST_ANY' in names and 'SQ_WAVE_CYCLES' in names:
+        derived.append(('wait %', None))
+    print('| kernel | ' + ' | '.join(names + [d for d, _ in derived]) + ' |')
+    print('|---|' + '---|' * (len(names) + len(derived)))
     for lab in chain_names(a.npri, a.ntrees):
-        if lab in acc:
-            print('| %s | ' % lab + ' | '.join('%.4g' % (sum(acc[lab][c]) / len(acc[lab][c])) for c in names) + ' |')
+        if lab not in acc:
+            continue
+        avg = {c: sum(acc[lab][c]) / len(acc[lab][c]) for c in names}
+        cells = ['%.4g' % avg[c] for c in names]
+        for d, c in derived:
+            if c is not None:
+                cells.append('%.1f' % (avg[c] / avg['SQ_WAVES']))
+            else:
+                cells.append('%.0f' % (100.0 * avg['SQ_WAIT_INST_ANY'] / avg['SQ_WAVE_CYCLES']))
+        print('| %s | ' % lab + ' | '.join(cells) + ' |')
 
 
 if __name__ == '__main__':
