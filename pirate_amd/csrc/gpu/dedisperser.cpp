@@ -196,7 +196,7 @@ void GpuDedisperser::validate_gpu_limits()
     PDD_CHECK(nt_in % 4 == 0, "GpuDedisperser: nt_in must be a multiple of 4");
     for (long p = 0; p < npri; p++) {
         long Tp = nt_in >> p;
-        PDD_CHECK(Tp % 64 == 0, "GpuDedisperser: this build needs nt_in/2^p to be a multiple of 64 for every primary tree p (nt_in="
+        PDD_CHECK(Tp % 32 == 0, "GpuDedisperser: this build needs nt_in/2^p to be a multiple of 32 for every primary tree p (nt_in="
                   << nt_in << ", p=" << p << ")");
         PDD_CHECK(plan->stage1_dd_rank[p] <= kGpuMaxDD && plan->stage1_dd_rank[p] >= 1,
                   "GpuDedisperser: stage-1 dd_rank " << plan->stage1_dd_rank[p] << " not supported by this build (max " << kGpuMaxDD << ")");
@@ -217,7 +217,6 @@ void GpuDedisperser::validate_gpu_limits()
             per_pass[T.stag[6 * k]]++;
         for (int c : per_pass)
             PDD_CHECK(c <= 16, "GpuDedisperser: tree " << it << " has more than 16 staggered subband pairs in one pass (not supported by this build)");
-        PDD_CHECK(t.nt_ds >= T.tpad + T.flmax, "GpuDedisperser: tree " << it << ": chunk too short for peak-finder history + final lags");
         PDD_CHECK(t.pf.time_downsampling <= 512, "GpuDedisperser: time_downsampling too large");
         PDD_CHECK(pf_config_supported(T.M, T.tpad, T.nlev, int(t.Dcore), int(t.pf.time_downsampling), t.nt_ds),
                   "GpuDedisperser: tree " << it << ": peak-finder shape not supported (M=" << T.M << ")");

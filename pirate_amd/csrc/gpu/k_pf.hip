@@ -349,15 +349,24 @@ k_pf(PfArgs a)
         }
     }
 
-    // ---- rotate rows for the next chunk: positions [Tp, Tp + tpad + flmax) -> [0, ...)
-    // (all staging loads completed before the last barrier; Tp >= tpad + flmax is checked
-    // on the host, so no position is both read and written here)
+    // ---- rotate rows for the next chunk: positions [Tp, Tp + tpad + flmax) -> [0, ...),
+    // i.e. the rows shift down by Tp. All staging loads completed before the last barrier.
+    // Done in column blocks of width min(Tp, tpad + flmax) in ascending order, with a
+    // barrier between blocks: within a block the positions read ([c0 + Tp, ...)) and
+    // written ([c0, c0 + width)) are disjoint, and a later block reads only positions at or
+    // above every position an earlier block wrote.
     const int nrot = a.tpad + a.flmax;
-    for (int idx = tid; idx < nstreams * nrot; idx += kPfThreads) {
-        const int s = idx / nrot, i = idx - s * nrot;
-        Tst *row = rows + long(s) * a.rowlen;
-        const Tst v = row[a.Tp + i];
-        row[i] = v;
+    const int wcol = min(a.Tp, nrot);
+    for (int c0 = 0; c0 < nrot; c0 += wcol) {
+        const int nc = min(wcol, nrot - c0);
+        for (int idx = tid; idx < nstreams * nc; idx += kPfThreads) {
+            const int s = idx / nc, i = c0 + (idx - s * nc);
+            Tst *row = rows + long(s) * a.rowlen;
+            const Tst v = row[a.Tp + i];
+            row[i] = v;
+        }
+        if (c0 + wcol < nrot)
+            __syncthreads();
     }
 }
 

@@ -11,6 +11,9 @@
 // histories (D/2 pairs x 2^r samples at pass r) live in LDS; passes alternate a read
 // phase (all lags resolved from tile or history into registers) and a write phase.
 // Histories persist across chunks in an HBM state buffer (loaded/saved once per chunk).
+// A chunk whose length is a multiple of 32 but not of 64 ends in a half tile (nv = 32
+// valid samples): its emits stop at nv, and each pass history is refilled with the last
+// 2^r valid samples, from the tile and, where 2^r > nv, from the older history.
 //
 // Stage 1 writes its output rows to the lag ring (per-row circular buffers whose
 // capacity covers the row's stage-2 lag). Stage 2 reads ring rows at their full lag
@@ -40,7 +43,7 @@ __device__ __forceinline__ int hist_off(int np, int r) { return np * ((1 << r) -
 // One in-place pass. If emit != nullptr, the outputs of main pairs with emit[p] >= 0
 // are also sent to streams emit[p] (out0) and emit[p]+1 (out1) through 'sink'.
 template<int DD, typename Sink>
-__device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const int *emit, Sink &sink)
+__device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const int *emit, Sink &sink, int nv)
 {
     using TD = TreeDims<DD>;
     const int tid = threadIdx.x;
@@ -62,7 +65,9 @@ __device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const
             const float xb = (t >= L + 1) ? tile[e * kTT + t - L - 1] : h[p * half + half + t - L - 1];
             o0[i] = kRsqrt2 * (xa + xo);
             o1[i] = kRsqrt2 * (xb + xo);
-            sv[i] = (t >= kTT - half) ? tile[e * kTT + t] : 0.0f;
+            // new history slot t - (kTT - half) holds time nv - half + (that slot)
+            const int ts = t - (kTT - nv);
+            sv[i] = (t < kTT - half) ? 0.0f : (ts >= 0) ? tile[e * kTT + ts] : h[p * half + half + ts];
         }
     }
     sink.read_phase_extra(r);
@@ -94,95 +99,6 @@ __device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const
 
 
 // ---------------------------------------------------------------------------------------
-// Stage 1
-
-
-long stage1_state_nelts_per_tree(int dd) { return (1L << (dd - 1)) * ((1L << dd) - 1); }
-
-struct NoSink {
-    __device__ void read_phase_extra(int) { }
-    __device__ void write_phase_extra(int) { }
-    __device__ void emit(int, int, float) { }
-};
-
-template<typename Tst, int DD>
-__global__ void __launch_bounds__(kTreeThreads)
-k_stage1(Stage1Args a)
-{
-    using TD = TreeDims<DD>;
-    extern __shared__ float smem[];
-    float *tile = smem;
-    float *hist = smem + TD::D * kTT;
-
-    const int ia = blockIdx.x, b = blockIdx.y;
-    const int tid = threadIdx.x;
-    float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(ia) * TD::HIST;
-
-    for (int i = tid; i < TD::HIST; i += kTreeThreads)
-        hist[i] = st[i];
-
-    const Tst *in = reinterpret_cast<const Tst *>(a.in) + long(b) * a.in_bstride + long(ia) * TD::D * a.Tp;
-    Tst *ring = reinterpret_cast<Tst *>(a.ring.base) + long(a.beam0 + b) * a.ring.bstride;
-    NoSink sink;
-
-    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
-        for (int idx = tid; idx < TD::D * kTT; idx += kTreeThreads) {
-            const int row = idx / kTT, t = idx % kTT;
-            tile[idx] = ld_st(in + long(row) * a.Tp + t0 + t);
-        }
-        __syncthreads();
-
-#pragma unroll 1
-        for (int r = 0; r < DD; r++)
-            tree_pass<DD>(r, tile, hist, nullptr, sink);
-
-        // Output row i (bit-reversed stage-1 DM) -> ring row (ia, i). One wave per row
-        // segment; ring capacities are multiples of 64, so a tile never wraps.
-        const long tau0 = a.chunk * a.Tp + t0;
-        const int lane = tid & 63;
-        for (int row = __builtin_amdgcn_readfirstlane(tid >> 6); row < TD::D; row += kTreeThreads / 64) {
-            const long rr = long(ia) * TD::D + row;
-            const long cap = a.ring.row_cap[rr];
-            const long pos = tau0 % cap;
-            st_st(ring + a.ring.row_off[rr] + pos + lane, tile[row * kTT + lane]);
-        }
-        __syncthreads();
-    }
-
-    for (int i = tid; i < TD::HIST; i += kTreeThreads)
-        st[i] = hist[i];
-}
-
-template<typename Tst>
-static void launch_stage1_t(const Stage1Args &a, hipStream_t st)
-{
-    dim3 grid(1 << a.amb, a.B);
-    dim3 block(kTreeThreads);
-#define PDD_S1(DDV)                                                                           \
-    case DDV: {                                                                               \
-        size_t shm = sizeof(float) * (TreeDims<DDV>::D * kTT + TreeDims<DDV>::HIST);          \
-        hipFuncSetAttribute((const void *) k_stage1<Tst, DDV>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)); \
-        hipLaunchKernelGGL((k_stage1<Tst, DDV>), grid, block, shm, st, a);                    \
-        break;                                                                                \
-    }
-    switch (a.dd) {
-        PDD_S1(1) PDD_S1(2) PDD_S1(3) PDD_S1(4) PDD_S1(5) PDD_S1(6) PDD_S1(7)
-        default: break;
-    }
-#undef PDD_S1
-}
-
-void launch_stage1(Dtype dt, const Stage1Args &a, hipStream_t st)
-{
-    switch (dt) {
-        case Dtype::float32: launch_stage1_t<float>(a, st); break;
-        case Dtype::float16: launch_stage1_t<__half>(a, st); break;
-        case Dtype::bfloat16: launch_stage1_t<__hip_bfloat16>(a, st); break;
-    }
-}
-
-
-// ---------------------------------------------------------------------------------------
 // Stage 2
 
 
@@ -205,7 +121,7 @@ struct Stage2Sink {
     Tst *sbb;            // sb + beam offset
     const int *stag, *fl;
     int M, rowlen, tpad;
-    int dpf_base, t0;
+    int dpf_base, t0, nv;     // nv: valid samples of the current tile
     const int *stag_begin;
     float *tile, *shist;
 
@@ -216,6 +132,8 @@ struct Stage2Sink {
 
     __device__ __forceinline__ void emit(int s, int t, float v)
     {
+        if (t >= nv)
+            return;
         const int pf_dm = s / M, m = s - pf_dm * M;
         const long row = long(dpf_base + pf_dm) * M + m;
         st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
@@ -238,7 +156,8 @@ struct Stage2Sink {
                 const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
                 so0[q] = kRsqrt2 * (xa + xo);
                 so1[q] = kRsqrt2 * (xb + xo);
-                ssv[q] = tile[row0 * kTT + t];
+                const int ts = t - (kTT - nv);
+                ssv[q] = (t < kTT - half) ? 0.0f : (ts >= 0) ? tile[row0 * kTT + ts] : h[half + ts];
             }
         }
     }
@@ -293,13 +212,16 @@ k_stage2(Stage2Args a)
     sink.tpad = a.tpad;
     sink.dpf_base = dpf_base;
     sink.t0 = 0;
+    sink.nv = kTT;
     sink.tile = tile;
     sink.shist = shist;
     sink.stag_begin = a.stag_begin;
     const int r_lvl0 = DD - a.pf_rank;
 
     for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
+        const int nv = min(kTT, a.Tp - t0);
         sink.t0 = t0;
+        sink.nv = nv;
         const long tau0 = a.chunk * a.Tp + t0;
         const int lane = tid & 63;
         for (int fc = __builtin_amdgcn_readfirstlane(tid >> 6); fc < TD::D; fc += kTreeThreads / 64) {
@@ -320,7 +242,7 @@ k_stage2(Stage2Args a)
                     sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
                 }
             }
-            tree_pass<DD>(r, tile, hist, a.main_emit, sink);
+            tree_pass<DD>(r, tile, hist, a.main_emit, sink, nv);
         }
         if (r_lvl0 == DD) {
             for (int idx = tid; idx < a.nlvl0 * kTT; idx += kTreeThreads) {

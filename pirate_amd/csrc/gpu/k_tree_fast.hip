@@ -20,6 +20,13 @@
 // (kept in registers: the per-pass history). Between the phases, rows go through an LDS delay
 // line per row (length lag + 64, shifted down by 64 after each read). Histories and the
 // delay-line contents persist across chunks in an HBM state buffer.
+//
+// Chunks whose length is a multiple of 32 but not of 64 end in a half tile: its loads past
+// the chunk read zeros, its lanes 32..63 store nothing, the delay lines shift down by 32,
+// and the saved pass histories are taken from lanes below 32. Garbage in lanes 32..63 never
+// reaches a lower lane, since every lag moves data to later times (higher lanes).
+
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -129,8 +136,10 @@ __device__ __forceinline__ void load_hist(float (&h)[Sub<K>::NH], const float *s
         h[0] = 0.0f;
 }
 
+// Saves the last 2^r samples of each pass history; 'hi' = valid lanes of the last tile
+// (64, or 32 for a half tile).
 template<int K>
-__device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *dst, int lane)
+__device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *dst, int lane, int hi)
 {
     constexpr int NP = Sub<K>::NP;
 #pragma unroll
@@ -138,8 +147,8 @@ __device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *d
         const int n = 1 << r;
 #pragma unroll
         for (int q = 0; q < NP; q++)
-            if (lane >= 64 - n)
-                dst[Sub<K>::hoff(r, q) + lane - (64 - n)] = h[r * NP + q];
+            if (lane >= hi - n && lane < hi)
+                dst[Sub<K>::hoff(r, q) + lane - (hi - n)] = h[r * NP + q];
     }
 }
 
@@ -175,6 +184,26 @@ struct PhaseB {
             p[0] = p[64];
             if (L > 64)          // wave-uniform
                 p[64] = p[128];
+        }
+    }
+
+    // Half tile (32 valid lanes): the same read, then a shift down by 32. Here lane t's
+    // store lands where lane t - 32 loads, so every load of the wave is issued before any
+    // store (LDS instructions of one wave execute in order).
+    __device__ __forceinline__ void read_half(float *lbuf, float (&y)[F::NRB], int lane) const
+    {
+#pragma unroll
+        for (int h = 0; h < F::NRB; h++) {
+            float *p = lbuf + base + h * step;
+            const int L = (F::NRB - 1 - h) * bj;
+            y[h] = p[0];
+            const float s0 = p[32];
+            const float s1 = (L > 64) ? p[96] : 0.0f;
+            __builtin_amdgcn_wave_barrier();
+            p[0] = s0;
+            if (L > 64)
+                p[64] = s1;
+            __builtin_amdgcn_wave_barrier();
         }
     }
 
@@ -296,70 +325,99 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
         dst = reinterpret_cast<Tst *>(a.sb) + gbeam * a.sb_bstride;
     }
 
-    // phase-A input rows of tile t0 (issued one tile ahead)
+    // phase-A input rows of tile t0 (issued one tile ahead); PART: a half tile, whose lanes
+    // past the chunk end read zeros
     float xa[F::SA][F::NRA];
-    auto load_a = [&](int t0) {
+    auto load_a = [&](int t0, auto part) {
+        constexpr bool PART = decltype(part)::value;
+        const bool live = !PART || (t0 + lane < a.Tp);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             const int h = wid + s * F::NW;
 #pragma unroll
             for (int l = 0; l < F::NRA; l++) {
                 if constexpr (!STAGE2) {
-                    xa[s][l] = ld_st(src + long(h * F::NRA + l) * a.Tp + t0 + lane);
+                    xa[s][l] = live ? ld_st(src + long(h * F::NRA + l) * a.Tp + t0 + lane) : 0.0f;
                 } else {
                     const int4 d = idesc[h * F::NRA + l];
                     const int tl = t0 + lane;
-                    xa[s][l] = ld_st(src + (d.x + tl - ((tl >= d.y) ? d.z : 0)));
+                    xa[s][l] = live ? ld_st(src + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : 0.0f;
                 }
             }
         }
     };
+    using Full = std::false_type;
+    using Half = std::true_type;
+    const int nfull = a.Tp >> 6;
+    const bool has_half = (a.Tp & 63) != 0;
+    // Stage-1 ring positions are multiples of 64 when the chunk is, so a full tile never
+    // wraps and the wrap test can use the tile start; otherwise it is made per lane.
+    const int wrap_lane = has_half ? lane : 0;
     // Stage 1 (and stage 2 below rank 7) issues the next tile's phase-A loads right after
     // this tile's phase A, so they are in flight during phase B; for rank-7 stage 2 the extra
     // registers would spill, and measured slower (1.31 vs 1.19 ms on chord16k).
     constexpr bool PREFETCH = !STAGE2 || DD <= 6;
-    if constexpr (PREFETCH)
-        load_a(0);
-    for (int t0 = 0; t0 < a.Tp; t0 += 64) {
+
+    auto tile_step = [&](int t0, auto half) {
+        constexpr bool HALF = decltype(half)::value;
         if constexpr (!PREFETCH)
-            load_a(t0);
+            load_a(t0, half);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             sub_passes<F::K1>(xa[s], ha[s], lane);
             phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
         }
         __syncthreads();
-        if constexpr (PREFETCH) {
-            if (t0 + 64 < a.Tp)
-                load_a(t0 + 64);
+        if constexpr (PREFETCH && !HALF) {
+            if (t0 + 128 <= a.Tp)
+                load_a(t0 + 64, Full{});
+            else if (t0 + 64 < a.Tp)
+                load_a(t0 + 64, Half{});
         }
 
+        const bool live = !HALF || (t0 + lane < a.Tp);
+        const int tw = t0 + wrap_lane;
 #pragma unroll
         for (int s = 0; s < F::SB; s++) {
             float y[F::NRB];
-            pb[s].read(lbuf, y, lane);
+            if constexpr (HALF)
+                pb[s].read_half(lbuf, y, lane);
+            else
+                pb[s].read(lbuf, y, lane);
             sub_passes<F::K2>(y, hb[s], lane);
 #pragma unroll
             for (int hp = 0; hp < F::NRB; hp++) {
                 const int4 d = odesc[hp * F::NRA + pb[s].j];
                 if constexpr (!STAGE2) {
-                    // ring capacities and positions are multiples of 64: a tile never wraps
-                    st_st(dst + (d.x + t0 - ((t0 >= d.y) ? d.z : 0) + lane), y[hp]);
+                    if (live)
+                        st_st(dst + (d.x + t0 - ((tw >= d.y) ? d.z : 0) + lane), y[hp]);
                 } else {
-                    if (d.x >= 0)
+                    if (d.x >= 0 && live)
                         st_st(dst + (long(d.x) + t0 + lane), y[hp]);
                 }
             }
         }
         __syncthreads();
-    }
+    };
 
+    if constexpr (PREFETCH) {
+        if (nfull > 0)
+            load_a(0, Full{});
+        else
+            load_a(0, Half{});
+    }
+    for (int it = 0; it < nfull; it++)
+        tile_step(it * 64, Full{});
+    if (has_half)
+        tile_step(nfull * 64, Half{});
+
+    const int hi = has_half ? (a.Tp & 63) : 64;
 #pragma unroll
     for (int s = 0; s < F::SA; s++)
-        save_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane);
+        save_hist<F::K1>(ha[s], st + (wid + s * F::NW) * SA_::NST, lane, hi);
 #pragma unroll
     for (int s = 0; s < F::SB; s++) {
-        save_hist<F::K2>(hb[s], st_b + pb[s].j * SB_::NST, lane);
+        save_hist<F::K2>(hb[s], st_b + pb[s].j * SB_::NST, lane, hi);
         pb[s].template residue<false>(lbuf, st_res, lane);
     }
 }

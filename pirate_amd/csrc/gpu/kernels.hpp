@@ -53,8 +53,6 @@ struct Stage1Args {
     int B, amb, dd, Tp, beam0;
     long chunk;               // chunk index (ring time = chunk*Tp + t)
 };
-long stage1_state_nelts_per_tree(int dd);
-void launch_stage1(Dtype dt, const Stage1Args &a, hipStream_t st);
 
 
 // ---- stage-2 tree + subband extraction -> peak-finder input rows ----

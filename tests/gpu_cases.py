@@ -37,6 +37,17 @@ CASES = {
     'wide_ch': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=8,
                     time_samples_per_chunk=256, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
                     frequency_subband_counts=[1], primary_trees=[pt(0, 8, 16, 16), pt(0, 4, 16, 16)]),
+    # float32 chunks that are a multiple of 32 but not of 64 samples: primary tree 0 (192
+    # samples) ends in a full tile, primary tree 1 (96 samples) in a half tile; subband
+    # levels with staggered streams put stage 2 on the general kernel.
+    'half_tile': dict(BASE, zone_nfreq=[300, 200], zone_freq_edges=[400, 600, 800], toplevel_tree_rank=8,
+                      time_samples_per_chunk=192, beams_per_gpu=2, beams_per_batch=1, num_active_batches=1,
+                      frequency_subband_counts=[2, 2, 1], primary_trees=[pt(1, 8, 16, 16), pt(0, 4, 16, 16)]),
+    # Chunks shorter than the peak finder's history + final lags (64 samples, max_width 32:
+    # 64 history samples), on a rank-9 tree whose stage 2 runs on the register kernel.
+    'short_chunk': dict(BASE, zone_nfreq=[512], zone_freq_edges=[400, 800], toplevel_tree_rank=9,
+                        time_samples_per_chunk=64, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
+                        frequency_subband_counts=[0, 3, 1], primary_trees=[pt(0, 32, 32, 64), pt(1, 16, 32, 32)]),
     # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,

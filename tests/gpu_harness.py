@@ -16,16 +16,18 @@ def random_weights(rng, dd, itree):
     return w.astype(np.float32)
 
 
-def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, variances=False):
+def run_and_compare(cfg_dict, nchunks, seed=0, input_kind='uniform', log=print, variances=False, nbatches_out=0,
+                    nbatches_wt=0):
     """Returns a list of per-(seq, tree) mismatch records (empty if bit-exact). With
     variances=True the peak finder's float64 out_var is compared too (exactly). With a 16-bit
-    dtype in cfg_dict, the oracle rounds to that type where the GPU chain stores it."""
+    dtype in cfg_dict, the oracle rounds to that type where the GPU chain stores it.
+    nbatches_out / nbatches_wt: output and weight ring depths (0 = the dedisperser's default)."""
     import torch
     import pirate_amd as pa
 
     cfg = pa.DedispersionConfig.from_dict(cfg_dict)
     plan = pa.DedispersionPlan(cfg)
-    dd = pa.GpuDedisperser(plan, enable_variances=variances)
+    dd = pa.GpuDedisperser(plan, enable_variances=variances, nbatches_out=nbatches_out, nbatches_wt=nbatches_wt)
     dd.allocate()
     storage = cfg_dict['dtype']
     ref = orc.Dedisperser(oracle_config(cfg_dict), 1, enable_variances=variances, storage=storage)
