@@ -262,7 +262,9 @@ long fast_tree_state_nelts(int dd)
 //   position wraps (capacity - position), z = capacity. Offsets are 32-bit (checked on host).
 //   peak-finder row (stage 2 output): x = element offset of (row, tpad + final lag) or -1.
 
-template<typename Tst, int DD, bool STAGE2, typename Args>
+// HC: the chunk may end in a half tile (Tp a multiple of 32, not of 64); launched only for
+// such chunks, so the common case keeps tile-uniform ring-wrap tests and no half-tile code.
+template<typename Tst, int DD, bool STAGE2, bool HC, typename Args>
 __device__ __forceinline__ void fast_tree_body(const Args &a)
 {
     using F = FastDims<DD>;
@@ -349,10 +351,7 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     using Full = std::false_type;
     using Half = std::true_type;
     const int nfull = a.Tp >> 6;
-    const bool has_half = (a.Tp & 63) != 0;
-    // Stage-1 ring positions are multiples of 64 when the chunk is, so a full tile never
-    // wraps and the wrap test can use the tile start; otherwise it is made per lane.
-    const int wrap_lane = has_half ? lane : 0;
+    const bool has_half = HC && (a.Tp & 63) != 0;
     // Stage 1 (and stage 2 below rank 7) issues the next tile's phase-A loads right after
     // this tile's phase A, so they are in flight during phase B; for rank-7 stage 2 the extra
     // registers would spill, and measured slower (1.31 vs 1.19 ms on chord16k).
@@ -360,7 +359,7 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
 
     auto tile_step = [&](int t0, auto half) {
         constexpr bool HALF = decltype(half)::value;
-        if constexpr (!PREFETCH)
+        if constexpr (!PREFETCH || HALF)      // a half tile is never prefetched
             load_a(t0, half);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
@@ -371,12 +370,12 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
         if constexpr (PREFETCH && !HALF) {
             if (t0 + 128 <= a.Tp)
                 load_a(t0 + 64, Full{});
-            else if (t0 + 64 < a.Tp)
-                load_a(t0 + 64, Half{});
         }
 
         const bool live = !HALF || (t0 + lane < a.Tp);
-        const int tw = t0 + wrap_lane;
+        // Stage-1 ring positions are multiples of 64 when the chunk is, so a tile never
+        // wraps and the wrap test uses the tile start; with half tiles it is made per lane.
+        const int tw = HC ? (t0 + lane) : t0;
 #pragma unroll
         for (int s = 0; s < F::SB; s++) {
             float y[F::NRB];
@@ -403,13 +402,13 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     if constexpr (PREFETCH) {
         if (nfull > 0)
             load_a(0, Full{});
-        else
-            load_a(0, Half{});
     }
     for (int it = 0; it < nfull; it++)
         tile_step(it * 64, Full{});
-    if (has_half)
-        tile_step(nfull * 64, Half{});
+    if constexpr (HC) {
+        if (has_half)
+            tile_step(nfull * 64, Half{});
+    }
 
     const int hi = has_half ? (a.Tp & 63) : 64;
 #pragma unroll
@@ -422,18 +421,18 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     }
 }
 
-template<typename Tst, int DD>
+template<typename Tst, int DD, bool HC>
 __global__ void __launch_bounds__(FastDims<DD>::NW * 64) __attribute__((amdgpu_waves_per_eu(FastDims<DD>::WAVES_PER_EU)))
 k_stage1_fast(Stage1Args a)
 {
-    fast_tree_body<Tst, DD, false>(a);
+    fast_tree_body<Tst, DD, false, HC>(a);
 }
 
-template<typename Tst, int DD>
+template<typename Tst, int DD, bool HC>
 __global__ void __launch_bounds__(FastDims<DD>::NW * 64) __attribute__((amdgpu_waves_per_eu(FastDims<DD>::WAVES_PER_EU)))
 k_stage2_fast(Stage2Args a)
 {
-    fast_tree_body<Tst, DD, true>(a);
+    fast_tree_body<Tst, DD, true, HC>(a);
 }
 
 
@@ -443,8 +442,12 @@ static void launch_stage1_fast_t(const Stage1Args &a, hipStream_t st)
 #define PDD_F1(DDV)                                                                          \
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
-        hipLaunchKernelGGL((k_stage1_fast<Tst, DDV>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
-                           F::LDS_BYTES, st, a);                            \
+        if (a.Tp & 63)                                                                       \
+            hipLaunchKernelGGL((k_stage1_fast<Tst, DDV, true>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
+                               F::LDS_BYTES, st, a);                                         \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_stage1_fast<Tst, DDV, false>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
+                               F::LDS_BYTES, st, a);                                         \
         break;                                                                               \
     }
     switch (a.dd) {
@@ -460,8 +463,12 @@ static void launch_stage2_fast_t(const Stage2Args &a, hipStream_t st)
 #define PDD_F2(DDV)                                                                          \
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
-        hipLaunchKernelGGL((k_stage2_fast<Tst, DDV>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
-                           F::LDS_BYTES, st, a);                            \
+        if (a.Tp & 63)                                                                       \
+            hipLaunchKernelGGL((k_stage2_fast<Tst, DDV, true>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
+                               F::LDS_BYTES, st, a);                                         \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_stage2_fast<Tst, DDV, false>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
+                               F::LDS_BYTES, st, a);                                         \
         break;                                                                               \
     }
     switch (a.dd) {
