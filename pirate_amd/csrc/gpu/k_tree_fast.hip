@@ -93,10 +93,14 @@ struct FastDims {
 };
 
 // Stream value at time t - L (L = 0..64): cur at lane t - L, or prev (the previous tile)
-// at lane 64 + t - L when t < L.
+// at lane 64 + t - L when t < L. L is a compile-time constant after unrolling; L = 1 (about
+// half of all shifts) is a whole-wave rotate by one lane in the VALU (DPP wave_ror:1, lane t
+// reads lane t - 1 mod 64), the other lags a ds_bpermute through the LDS crossbar.
 __device__ __forceinline__ float lane_shift(float cur, float prev, int L, int lane)
 {
     const float w = (lane >= 64 - L) ? prev : cur;
+    if (L == 1)
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x13C, 0xf, 0xf, false));
     return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane - L) & 63) << 2, __float_as_int(w)));
 }
 
