@@ -15,7 +15,8 @@ namespace pirate_amd {
             throw pdd_error(std::string("HIP error: ") + hipGetErrorString(_e) + " in " + #expr); \
     } while (0)
 
-static constexpr int kGpuMaxDD = 7;
+static constexpr int kGpuMaxDD = 8;          // register-tree kernels (stage 1, final-pass stage 2)
+static constexpr int kGpuMaxDDGeneral = 7;   // LDS-tile stage 2 (subband searches)
 static constexpr long kLdsBytesPerCU = 160 * 1024;
 
 
@@ -210,8 +211,13 @@ void GpuDedisperser::validate_gpu_limits()
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
         PDD_CHECK(t.dd_rank <= kGpuMaxDD, "GpuDedisperser: stage-2 dd_rank " << t.dd_rank << " not supported by this build (max " << kGpuMaxDD << ")");
-        PDD_CHECK(stage2_lds_bytes(t.dd_rank, T.stag_hist_nelts) <= kLdsBytesPerCU,
-                  "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
+        if (!T.fast) {
+            PDD_CHECK(t.dd_rank <= kGpuMaxDDGeneral, "GpuDedisperser: tree " << it << ": stage-2 dd_rank " << t.dd_rank
+                      << " with this subband search (streams before the last pass) is not supported by this build (max "
+                      << kGpuMaxDDGeneral << "; rank 8 needs subband counts [0,...,0,1])");
+            PDD_CHECK(stage2_lds_bytes(t.dd_rank, T.stag_hist_nelts) <= kLdsBytesPerCU,
+                      "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
+        }
         std::vector<int> per_pass(t.dd_rank, 0);
         for (int k = 0; k < T.nstag; k++)
             per_pass[T.stag[6 * k]]++;

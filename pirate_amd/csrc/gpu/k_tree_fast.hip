@@ -67,19 +67,46 @@ struct FastDims {
     static constexpr int T2 = NRB * (NRB - 1) / 2;
     static constexpr int SUMLAG = T2 * (NRA * (NRA - 1) / 2);
     static constexpr int NSTATE = NRB * Sub<K1>::NST + NRA * Sub<K2>::NST + SUMLAG;
-    static_assert((NRB - 1) * (NRA - 1) < 128, "delay-line shift handles lags below 128");
+    static constexpr int LMAX = (NRB - 1) * (NRA - 1);       // largest delay-line lag
+    static_assert(LMAX < 256, "delay-line shift handles lags below 256");
 
-    // Delay lines: row (h, j) starts at h*S + Q(j) and holds Lm(j) + 64 floats, where
-    // Lm(j) = (NRB-1)*bitrev(j) is the largest delay over h. Phase A writes a tile at
-    // h*S + Q(j) + Lm(j); phase B reads row (h, j) delayed by lag(h, j) at h*S + Q(j) +
-    // Lm(j) - lag(h, j) = Q(j) + h*(S + bitrev(j)).
+    // Delay lines. Row (h, j) holds the last lag(h, j) samples of phase-A output row (h, j)
+    // followed by the current tile: phase A writes the tile at a_off(h, j), phase B reads
+    // the delayed window at b_off(h, j) (= a_off - lag) and then shifts the line down by 64.
+    // Ranks up to 7 use a 2-D layout with a compile-time phase-A offset per (wave, j): row
+    // (h, j) starts at h*S + Q(j) and reserves Lm(j) + 64 floats, Lm(j) = (NRB-1)*bitrev(j)
+    // the largest lag over h. Rank 8 packs the rows (j-major, each row lag(h, j) + 64
+    // floats), which needs 120 KB instead of 176 KB; gfx950 gives one workgroup up to 160 KB.
+    static constexpr bool PACKED = (DD >= 8);
     static constexpr int Lm(int j) { return (NRB - 1) * brev_c(j, K1); }
     static constexpr int Q(int j) { return (j == 0) ? 0 : Q(j - 1) + Lm(j - 1) + 64; }
     static constexpr int S = Q(NRA);
-    static constexpr int DL_FLOATS = ((NRB * S + 128 + 3) / 4) * 4;   // + padding for the shift reads
+    static constexpr int G(int h) { return h * (NRB - 1) - h * (h - 1) / 2; }   // sum of (NRB-1-h') over h' < h
+    static constexpr int CQ(int j) { return (j == 0) ? 0 : CQ(j - 1) + brev_c(j - 1, K1) * T2 + 64 * NRB; }
+    static constexpr int DL_USED = PACKED ? CQ(NRA) : NRB * S;
+    static constexpr int DL_FLOATS = ((DL_USED + 256 + 3) / 4) * 4;   // + padding for the shift reads
     static constexpr int LDS_BYTES = 4 * DL_FLOATS + 2 * 16 * D;   // + two int4 row-descriptor tables
-    // rank 7: 62 KB of LDS allows 2 workgroups (16 waves) per CU, so aim for 128 VGPRs
+    // rank 7: 62 KB of LDS allows 2 workgroups (16 waves) per CU, so aim for 128 VGPRs;
+    // rank 8: one workgroup (8 waves, two sub-trees per wave and phase) per CU, 256 VGPRs
     static constexpr int WAVES_PER_EU = (DD == 7) ? 4 : 2;
+
+    __host__ __device__ static constexpr int lag_c(int h, int j) { return (NRB - 1 - h) * brev_c(j, K1); }
+    // phase-A write offset of row (h, j) (h = the wave's sub-tree, j unrolled)
+    __device__ static int a_off(int h, int j)
+    {
+        if constexpr (PACKED)
+            return CQ(j) + 64 * h + brev_c(j, K1) * G(h + 1);
+        else
+            return h * S + Q(j) + Lm(j);
+    }
+    // phase-B read offset of row (h, j) (j = the wave's sub-tree, h unrolled), bj = bitrev(j)
+    __device__ static int b_off(int h, int j, int bj)
+    {
+        if constexpr (PACKED)
+            return CQ(j) + 64 * h + bj * G(h);
+        else
+            return Q(j) + h * (S + bj);
+    }
 
     __device__ static int lag(int h, int j) { return (NRB - 1 - h) * brev_c(j, K1); }
     // residue of row (h, j) in the state buffer: rows j-major, h inner, lag(h, j) floats each
@@ -160,34 +187,49 @@ __device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *d
 template<int DD>
 struct PhaseB {
     using F = FastDims<DD>;
-    int j, bj, base, step;     // base = Q(j) + lane, step = S + bitrev(j)
+    int j, bj, cq;     // cq: offset of sub-tree j's rows in the packed layout (unused otherwise)
 
     __device__ void init(int jj, int lane)
     {
+        (void) lane;
         j = jj;
         bj = brev_c(j, F::K1);
         int q = 0;
         for (int k = 0; k < j; k++)
-            q += F::Lm(k) + 64;
-        base = q + lane;
-        step = F::S + bj;
+            q += F::PACKED ? (brev_c(k, F::K1) * F::T2 + 64 * F::NRB) : (F::Lm(k) + 64);
+        cq = q;
+    }
+
+    // start of row (h, j)'s delayed window (h compile-time after unrolling)
+    __device__ __forceinline__ int row(int h) const
+    {
+        if constexpr (F::PACKED)
+            return cq + 64 * h + bj * F::G(h);
+        else
+            return cq + h * (F::S + bj);
     }
 
     // Row (h, j) at the delayed time of this tile's lane, then shift the line down by 64
-    // (positions [64, 64 + lag) -> [0, lag)); in-order LDS access within the wave keeps the
-    // overlapping copy correct. Only positions below lag must be right: [lag, lag + 64) is
-    // the next tile's phase-A window, so lanes at or above lag may copy garbage (reads past
-    // the row stay inside the padded buffer), which keeps the copy branch-free.
+    // (positions [64, 64 + lag) -> [0, lag)); each lane copies its own column, in order.
+    // Only positions below lag must be right: [lag, lag + 64) is the next tile's phase-A
+    // window, so lanes at or above lag may copy garbage (reads past the row stay inside the
+    // padded buffer), which keeps the copy branch-free.
     __device__ __forceinline__ void read(float *lbuf, float (&y)[F::NRB], int lane) const
     {
 #pragma unroll
         for (int h = 0; h < F::NRB; h++) {
-            float *p = lbuf + base + h * step;
+            float *p = lbuf + row(h) + lane;
             const int L = (F::NRB - 1 - h) * bj;
             y[h] = p[0];
             p[0] = p[64];
             if (L > 64)          // wave-uniform
                 p[64] = p[128];
+            if constexpr (F::LMAX > 128) {
+                if (L > 128)
+                    p[128] = p[192];
+                if (L > 192)
+                    p[192] = p[256];
+            }
         }
     }
 
@@ -198,15 +240,26 @@ struct PhaseB {
     {
 #pragma unroll
         for (int h = 0; h < F::NRB; h++) {
-            float *p = lbuf + base + h * step;
+            float *p = lbuf + row(h) + lane;
             const int L = (F::NRB - 1 - h) * bj;
             y[h] = p[0];
             const float s0 = p[32];
             const float s1 = (L > 64) ? p[96] : 0.0f;
+            float s2 = 0.0f, s3 = 0.0f;
+            if constexpr (F::LMAX > 128) {
+                s2 = (L > 128) ? p[160] : 0.0f;
+                s3 = (L > 192) ? p[224] : 0.0f;
+            }
             __builtin_amdgcn_wave_barrier();
             p[0] = s0;
             if (L > 64)
                 p[64] = s1;
+            if constexpr (F::LMAX > 128) {
+                if (L > 128)
+                    p[128] = s2;
+                if (L > 192)
+                    p[192] = s3;
+            }
             __builtin_amdgcn_wave_barrier();
         }
     }
@@ -215,7 +268,7 @@ struct PhaseB {
     __device__ void residue(float *lbuf, float *res, int lane) const
     {
         for (int h = 0; h < F::NRB; h++) {
-            float *p = lbuf + base - lane + h * step;
+            float *p = lbuf + row(h);
             float *r = res + F::lagsum(h, j);
             const int L = F::lag(h, j);
             for (int q = lane; q < L; q += 64) {
@@ -232,10 +285,9 @@ template<int DD, int K1 = FastDims<DD>::K1>
 __device__ __forceinline__ void phase_a_write(float *lbuf, int h, const float (&x)[1 << K1], int lane)
 {
     using F = FastDims<DD>;
-    float *w = lbuf + h * F::S + lane;
 #pragma unroll
     for (int j = 0; j < F::NRA; j++)
-        w[F::Q(j) + F::Lm(j)] = x[j];
+        lbuf[F::a_off(h, j) + lane] = x[j];
 }
 
 }  // namespace
@@ -251,6 +303,7 @@ long fast_tree_state_nelts(int dd)
         case 5: return FastDims<5>::NSTATE;
         case 6: return FastDims<6>::NSTATE;
         case 7: return FastDims<7>::NSTATE;
+        case 8: return FastDims<8>::NSTATE;
         default: return 0;
     }
 }
@@ -369,6 +422,8 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
         for (int s = 0; s < F::SA; s++) {
             sub_passes<F::K1>(xa[s], ha[s], lane);
             phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
+            if constexpr (F::SA > 1 && DD >= 8)
+                __builtin_amdgcn_sched_barrier(0);    // one sub-tree at a time (register pressure)
         }
         __syncthreads();
         if constexpr (PREFETCH && !HALF) {
@@ -399,6 +454,8 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
                         st_st(dst + (long(d.x) + t0 + lane), y[hp]);
                 }
             }
+            if constexpr (F::SB > 1 && DD >= 8)
+                __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
     };
@@ -440,22 +497,32 @@ k_stage2_fast(Stage2Args a)
 }
 
 
+// Dynamic LDS above 64 KB (rank 8) must be allowed per kernel; gfx950 allows 160 KB.
+static void allow_lds(const void *fn, int bytes)
+{
+    if (bytes > 65536)
+        (void) hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 template<typename Tst>
 static void launch_stage1_fast_t(const Stage1Args &a, hipStream_t st)
 {
 #define PDD_F1(DDV)                                                                          \
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
-        if (a.Tp & 63)                                                                       \
+        if (a.Tp & 63) {                                                                     \
+            allow_lds((const void *) k_stage1_fast<Tst, DDV, true>, F::LDS_BYTES);             \
             hipLaunchKernelGGL((k_stage1_fast<Tst, DDV, true>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
                                F::LDS_BYTES, st, a);                                         \
-        else                                                                                 \
+        } else {                                                                             \
+            allow_lds((const void *) k_stage1_fast<Tst, DDV, false>, F::LDS_BYTES);            \
             hipLaunchKernelGGL((k_stage1_fast<Tst, DDV, false>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
                                F::LDS_BYTES, st, a);                                         \
+        }                                                                                    \
         break;                                                                               \
     }
     switch (a.dd) {
-        PDD_F1(1) PDD_F1(2) PDD_F1(3) PDD_F1(4) PDD_F1(5) PDD_F1(6) PDD_F1(7)
+        PDD_F1(1) PDD_F1(2) PDD_F1(3) PDD_F1(4) PDD_F1(5) PDD_F1(6) PDD_F1(7) PDD_F1(8)
         default: break;
     }
 #undef PDD_F1
@@ -467,16 +534,19 @@ static void launch_stage2_fast_t(const Stage2Args &a, hipStream_t st)
 #define PDD_F2(DDV)                                                                          \
     case DDV: {                                                                              \
         using F = FastDims<DDV>;                                                             \
-        if (a.Tp & 63)                                                                       \
+        if (a.Tp & 63) {                                                                     \
+            allow_lds((const void *) k_stage2_fast<Tst, DDV, true>, F::LDS_BYTES);             \
             hipLaunchKernelGGL((k_stage2_fast<Tst, DDV, true>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
                                F::LDS_BYTES, st, a);                                         \
-        else                                                                                 \
+        } else {                                                                             \
+            allow_lds((const void *) k_stage2_fast<Tst, DDV, false>, F::LDS_BYTES);            \
             hipLaunchKernelGGL((k_stage2_fast<Tst, DDV, false>), dim3(1 << a.amb, a.B), dim3(F::NW * 64), \
                                F::LDS_BYTES, st, a);                                         \
+        }                                                                                    \
         break;                                                                               \
     }
     switch (a.dd) {
-        PDD_F2(1) PDD_F2(2) PDD_F2(3) PDD_F2(4) PDD_F2(5) PDD_F2(6) PDD_F2(7)
+        PDD_F2(1) PDD_F2(2) PDD_F2(3) PDD_F2(4) PDD_F2(5) PDD_F2(6) PDD_F2(7) PDD_F2(8)
         default: break;
     }
 #undef PDD_F2

@@ -48,11 +48,24 @@ CASES = {
     'short_chunk': dict(BASE, zone_nfreq=[512], zone_freq_edges=[400, 800], toplevel_tree_rank=9,
                         time_samples_per_chunk=64, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
                         frequency_subband_counts=[0, 3, 1], primary_trees=[pt(0, 32, 32, 64), pt(1, 16, 32, 32)]),
+    # Rank-8 stage 2 (toplevel rank 15, as the reference's chime configs): 9 chunks of 4096
+    # samples cover the largest ring lag (255 * 127 = 32385 samples).
+    'rank15': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=15,
+                   time_samples_per_chunk=4096, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                   frequency_subband_counts=[0, 0, 0, 0, 1], primary_trees=[pt(0, 16, 64, 64)]),
+    # Rank-8 stage 1 and stage 2 (toplevel rank 16, as the reference's chord configs), two
+    # primary trees.
+    'rank16': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=16,
+                   time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                   frequency_subband_counts=[1], primary_trees=[pt(0, 8, 64, 64), pt(0, 4, 64, 64)]),
     # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                  frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
 }
+
+# Cases whose dense CPU checks (e.g. the analytic-variance oracle) would take minutes.
+LARGE = {'rank15', 'rank16'}
 
 # chord16k (bench workload): 16 beams x 16384 channels 300-1500 MHz, rank 14, four
 # primary trees (1/2/4/8 ms), chunks of 2048 samples, float32. Not fixed by the workload and
