@@ -135,6 +135,11 @@ int pdd_dedisperser_allocate(pdd_dedisperser *dd);
 /* Bytes of device memory held by the dedisperser. */
 int pdd_dedisperser_gpu_bytes(const pdd_dedisperser *dd, int64_t *nbytes);
 
+/* Bytes of pinned host memory held for lag-ring rows spilled to the host: rows whose
+ * smallest stage-2 lag spans more than config max_gpu_clag chunks (the reference's host
+ * ring zones, MegaRingbuf.hpp:45-58, GpuDedisperser.cpp:628-707); 0 when nothing spills. */
+int pdd_dedisperser_host_bytes(const pdd_dedisperser *dd, int64_t *nbytes);
+
 /* GpuDedisperser::acquire_input(seq_id, stream), GpuDedisperser.cpp:742-823.
  * *slot receives the device address of a contiguous (beams_per_batch, nfreq, nt_in) array
  * of the config dtype, valid until the matching release. */

@@ -57,6 +57,7 @@ _SIGS = {
     'pdd_dedisperser_destroy': (None, [vp]),
     'pdd_dedisperser_allocate': (ctypes.c_int, [vp]),
     'pdd_dedisperser_gpu_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+    'pdd_dedisperser_host_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
     'pdd_acquire_input': (ctypes.c_int, [vp, i64, vp, ctypes.POINTER(vp)]),
     'pdd_release_input_and_launch': (ctypes.c_int, [vp, i64, vp]),
     'pdd_acquire_output': (ctypes.c_int, [vp, i64, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),

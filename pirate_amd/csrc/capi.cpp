@@ -60,7 +60,7 @@ static int copy_string(const std::string &s, char *buf, size_t buflen, size_t *n
 extern "C" {
 
 const char *pdd_last_error(void) { return g_last_error.c_str(); }
-int pdd_abi_version(void) { return 2; }
+int pdd_abi_version(void) { return 3; }
 
 int pdd_config_from_yaml_file(const char *filename, pdd_config **out)
 {
@@ -260,6 +260,11 @@ int pdd_dedisperser_allocate(pdd_dedisperser *dd) { PDD_TRY_DD(dd, dd->d->alloca
 int pdd_dedisperser_gpu_bytes(const pdd_dedisperser *dd, int64_t *nbytes)
 {
     PDD_TRY({ PDD_CHECK(dd && nbytes, "null argument"); *nbytes = int64_t(dd->d->gpu_bytes_allocated); });
+}
+
+int pdd_dedisperser_host_bytes(const pdd_dedisperser *dd, int64_t *nbytes)
+{
+    PDD_TRY({ PDD_CHECK(dd && nbytes, "null argument"); *nbytes = int64_t(dd->d->host_bytes_allocated); });
 }
 
 int pdd_acquire_input(pdd_dedisperser *dd, int64_t seq_id, void *stream, void **slot)

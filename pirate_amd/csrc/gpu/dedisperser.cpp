@@ -60,7 +60,11 @@ GpuDedisperser::~GpuDedisperser()
     freeit(lds_state);
     freeit(grid_rows);
     freeit(grid_w);
-    for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &stage2_lags, &sb, &d_main_emit, &d_stag,
+    for (void *h : host_ring)
+        if (h)
+            (void) hipHostFree(h);
+    for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &ring1_row_off, &ring1_row_cap, &spill_rows,
+                    &stage2_lags, &sb, &d_main_emit, &d_stag,
                     &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_max, &out_argmax, &out_var})
         for (auto &a : *v)
             freeit(a);
@@ -302,39 +306,90 @@ void GpuDedisperser::allocate()
     }
 
     // Lag ring: per primary tree p and stage-1 output row (a1, j), a circular row whose
-    // capacity covers the largest stage-2 lag of that row plus one chunk.
+    // capacity covers the largest stage-2 lag of that row plus one chunk; rows whose
+    // smallest lag spans more than max_gpu_clag chunks keep k whole chunks of it in host
+    // memory (see SpillGroup in the header).
     rings.resize(npri);
     ring_row_off.resize(npri);
     ring_row_cap.resize(npri);
+    ring1_row_off.resize(npri);
+    ring1_row_cap.resize(npri);
     ring_bstride.resize(npri);
+    spill_groups.assign(npri, {});
+    spill_k.assign(npri, {});
+    nspill.assign(npri, 0);
+    outbox_off.assign(npri, 0);
+    host_bstride.assign(npri, 0);
+    spill_rows.resize(npri);
+    host_ring.assign(npri, nullptr);
     s1_state.resize(npri);
     s1_state_bstride.resize(npri);
+    const long max_gpu_clag = plan->config.max_gpu_clag;
     for (long p = 0; p < npri; p++) {
         const int dd1 = int(plan->stage1_dd_rank[p]), amb1 = int(plan->stage1_amb_rank[p]);
         const long Tp = nt_in >> p;
         const long nrows = pow2(dd1 + amb1);
-        std::vector<long> off(nrows), cap(nrows);
+        std::vector<long> off(nrows), cap(nrows), kr(nrows, 0);
         long tot = 0;
         for (long a1 = 0; a1 < pow2(amb1); a1++)
             for (long j = 0; j < pow2(dd1); j++) {
-                long maxlag = 0;
+                long maxlag = 0, minlag = -1;
                 for (long it = 0; it < ntrees; it++) {
                     const DedispersionTree &t = plan->trees[it];
                     if (t.primary_tree_index != p || a1 >= pow2(t.dd_rank))
                         continue;
-                    maxlag = std::max(maxlag, rb_lag(a1, j, t.amb_rank, t.dd_rank, p > 0));
+                    const long lag = rb_lag(a1, j, t.amb_rank, t.dd_rank, p > 0);
+                    maxlag = std::max(maxlag, lag);
+                    minlag = (minlag < 0) ? lag : std::min(minlag, lag);
                 }
                 long r = a1 * pow2(dd1) + j;
-                cap[r] = align_up(maxlag + Tp, 64);
+                if (minlag >= 0 && max_gpu_clag < (1L << 40))
+                    kr[r] = std::max(0L, minlag / Tp - max_gpu_clag);
+                cap[r] = align_up(maxlag - kr[r] * Tp + Tp, 64);
                 off[r] = tot;
                 tot += cap[r];
             }
-        ring_bstride[p] = align_up(tot, 64);
+        // Spilled rows in outbox order: grouped by k, ascending.
+        std::vector<long> order;
+        for (long r = 0; r < nrows; r++)
+            if (kr[r] > 0)
+                order.push_back(r);
+        std::stable_sort(order.begin(), order.end(), [&](long x, long y) { return kr[x] < kr[y]; });
+        nspill[p] = long(order.size());
+        outbox_off[p] = align_up(tot, 64);
+        std::vector<long> off1(off), cap1(cap);
+        std::vector<int> srows(order.size());
+        long hoff = 0;
+        for (size_t i = 0; i < order.size(); i++) {
+            const long r = order[i];
+            srows[i] = int(r);
+            off1[r] = outbox_off[p] + long(i) * Tp;
+            cap1[r] = Tp;
+            if (spill_groups[p].empty() || spill_groups[p].back().k != kr[r])
+                spill_groups[p].push_back(SpillGroup{kr[r], long(i), 0, 0});
+            spill_groups[p].back().nrows++;
+        }
+        for (SpillGroup &g : spill_groups[p]) {
+            g.host_off = hoff;
+            hoff += (g.k + 1) * g.nrows * Tp;
+        }
+        host_bstride[p] = align_up(hoff, 64);
+        spill_k[p] = kr;
+        ring_bstride[p] = align_up(outbox_off[p] + nspill[p] * Tp, 64);
         PDD_CHECK(ring_bstride[p] < (1L << 31), "GpuDedisperser: lag ring of primary tree " << p
                   << " exceeds 2^31 elements per beam (tree kernels use 32-bit row offsets)");
         rings[p] = dalloc(size_t(total_beams) * ring_bstride[p] * elt_bytes);
         ring_row_off[p] = up_long(off);
         ring_row_cap[p] = up_long(cap);
+        ring1_row_off[p] = up_long(off1);
+        ring1_row_cap[p] = up_long(cap1);
+        spill_rows[p] = up_int(srows);
+        if (nspill[p] > 0) {
+            const size_t hb = size_t(total_beams) * host_bstride[p] * elt_bytes;
+            HIP_CHECK(hipHostMalloc(&host_ring[p], hb, hipHostMallocDefault));
+            std::memset(host_ring[p], 0, hb);
+            host_bytes_allocated += hb;
+        }
         s1_state_bstride[p] = pow2(amb1) * fast_tree_state_nelts(int(dd1));
         s1_state[p] = dalloc(size_t(total_beams) * s1_state_bstride[p] * sizeof(float));
     }
@@ -358,9 +413,12 @@ void GpuDedisperser::allocate()
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
         std::vector<int> lags(pow2(t.dd_rank + t.amb_rank));
+        const std::vector<long> &kr = spill_k[t.primary_tree_index];
         for (long fc = 0; fc < pow2(t.dd_rank); fc++)
-            for (long j = 0; j < pow2(t.amb_rank); j++)
-                lags[fc * pow2(t.amb_rank) + j] = int(rb_lag(fc, j, t.amb_rank, t.dd_rank, t.primary_tree_index > 0));
+            for (long j = 0; j < pow2(t.amb_rank); j++) {
+                const long rr = fc * pow2(t.amb_rank) + j;
+                lags[rr] = int(rb_lag(fc, j, t.amb_rank, t.dd_rank, t.primary_tree_index > 0) - kr[rr] * t.nt_ds);
+            }
         stage2_lags[it] = up_int(lags);
         // Peak-finder input rows hold per-beam history (and final-lag "future" samples)
         // across chunks: one set of rows per beam of the GPU, not per batch.
@@ -443,8 +501,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.in = p ? lds_out[p].ptr : grid_buf.ptr;
         s.in_bstride = p ? (N / 2) * Tp : N * nt_in;
         s.ring.base = rings[p].ptr;
-        s.ring.row_off = (const long *) ring_row_off[p].ptr;
-        s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
+        s.ring.row_off = (const long *) ring1_row_off[p].ptr;     // spilled rows: the outbox
+        s.ring.row_cap = (const long *) ring1_row_cap[p].ptr;
         s.ring.bstride = ring_bstride[p];
         s.state = (float *) s1_state[p].ptr;
         s.state_bstride = s1_state_bstride[p];
@@ -459,6 +517,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         timed_begin(nm, st, ev);
         launch_stage1_fast(dtype, s, st);
         timed_end(nm, st, ev);
+        if (nspill[p] > 0)
+            spill_exchange(p, ichunk, beam0, st);
     }
     for (long it = 0; it < ntrees; it++) {
         const DedispersionTree &t = plan->trees[it];
@@ -538,6 +598,39 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         timed_end(nmp, st, ev);
     }
     HIP_CHECK(hipGetLastError());
+}
+
+
+// Host spill for primary tree p after stage 1 of (chunk, batch): outbox -> host slot
+// (chunk mod (k+1)), host slot ((chunk - k) mod (k+1)) -> outbox, outbox -> device ring rows
+// at ring time chunk * Tp. All on 'st', in this order, so the outbox is read before it is
+// refilled and stage 2 sees the rows appended. A slot not yet written holds zeros (the
+// stream before time 0).
+void GpuDedisperser::spill_exchange(long p, long ichunk, int beam0, hipStream_t st)
+{
+    const long Tp = nt_in >> p, B = beams_per_batch;
+    const size_t rb_pitch = size_t(ring_bstride[p]) * elt_bytes, hb_pitch = size_t(host_bstride[p]) * elt_bytes;
+    char *ring_b = (char *) rings[p].ptr + size_t(beam0) * rb_pitch;
+    char *host_b = (char *) host_ring[p] + size_t(beam0) * hb_pitch;
+    hipEvent_t ev;
+    timed_begin("spill_p" + std::to_string(p), st, ev);
+    for (const SpillGroup &g : spill_groups[p]) {
+        const size_t width = size_t(g.nrows) * Tp * elt_bytes;
+        char *dev = ring_b + size_t(outbox_off[p] + g.row0 * Tp) * elt_bytes;
+        char *host = host_b + size_t(g.host_off + (ichunk % (g.k + 1)) * g.nrows * Tp) * elt_bytes;
+        HIP_CHECK(hipMemcpy2DAsync(host, hb_pitch, dev, rb_pitch, width, B, hipMemcpyDeviceToHost, st));
+    }
+    for (const SpillGroup &g : spill_groups[p]) {
+        const size_t width = size_t(g.nrows) * Tp * elt_bytes;
+        const long slot = (((ichunk - g.k) % (g.k + 1)) + (g.k + 1)) % (g.k + 1);
+        char *dev = ring_b + size_t(outbox_off[p] + g.row0 * Tp) * elt_bytes;
+        char *host = host_b + size_t(g.host_off + slot * g.nrows * Tp) * elt_bytes;
+        HIP_CHECK(hipMemcpy2DAsync(dev, rb_pitch, host, hb_pitch, width, B, hipMemcpyHostToDevice, st));
+    }
+    launch_spill_scatter(rings[p].ptr, ring_bstride[p], int(elt_bytes), outbox_off[p], (const int *) spill_rows[p].ptr,
+                         nspill[p], (const long *) ring_row_off[p].ptr, (const long *) ring_row_cap[p].ptr, Tp, ichunk,
+                         beam0, int(B), st);
+    timed_end("spill_p" + std::to_string(p), st, ev);
 }
 
 

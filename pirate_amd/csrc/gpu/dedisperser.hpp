@@ -55,6 +55,7 @@ public:
     void allocate();
     bool is_allocated = false;
     size_t gpu_bytes_allocated = 0;
+    size_t host_bytes_allocated = 0;  // pinned host memory of spilled lag-ring rows
 
     // Returns the device address of the (B, nfreq, nt_in) input slot for seq_id.
     void *acquire_input(long seq_id, hipStream_t stream);
@@ -118,8 +119,23 @@ private:
     DeviceArray grid_buf;             // (B, 2^R, nt_in)
     std::vector<DeviceArray> lds_out; // [p] (B, 2^(R-1), nt_in >> p), p >= 1
     std::vector<DeviceArray> rings;   // [p]
-    std::vector<DeviceArray> ring_row_off, ring_row_cap;   // [p] device (long)
+    std::vector<DeviceArray> ring_row_off, ring_row_cap;   // [p] device (long), stage-2 (read) side
     std::vector<long> ring_bstride;
+
+    // Host spill of lag-ring rows (config max_gpu_clag). Row r of primary tree p whose
+    // smallest stage-2 lag spans more than max_gpu_clag chunks is delayed by k_r whole chunks
+    // in pinned host memory: stage 1 writes the row's chunk to an outbox at the end of the
+    // beam's ring region, the outbox goes to host slot (chunk mod (k_r+1)), the slot written
+    // k_r chunks earlier comes back into the outbox, and a scatter kernel appends it to the
+    // row's (shorter) device ring, which stage 2 then reads with lag - k_r * Tp. Rows are
+    // grouped by k_r, so each group is one 2-D copy per direction per batch.
+    struct SpillGroup { long k, row0, nrows, host_off; };
+    std::vector<std::vector<SpillGroup>> spill_groups;       // [p]
+    std::vector<std::vector<long>> spill_k;                  // [p][ring row] host delay in chunks
+    std::vector<long> nspill, outbox_off, host_bstride;      // [p]
+    std::vector<DeviceArray> spill_rows;                     // [p] device int: ring row of each outbox row
+    std::vector<DeviceArray> ring1_row_off, ring1_row_cap;   // [p] stage-1 (write) side
+    std::vector<void *> host_ring;                           // [p] pinned, (total_beams, host_bstride)
     std::vector<DeviceArray> stage2_lags;    // [itree] device int
     std::vector<DeviceArray> sb;      // [itree] (B, ndm_out, M, rowlen)
     std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin, d_out_stream;
@@ -150,6 +166,7 @@ private:
 
     DeviceArray dalloc(size_t nbytes);
     void check_stopped_locked();
+    void spill_exchange(long p, long ichunk, int beam0, hipStream_t st);
     void build_tables();
     void validate_gpu_limits();
 };

@@ -43,6 +43,44 @@ k_ringbuf_copy(uint4 *rb, const uint32_t *oct, long noct, int nbeams, unsigned l
     }
 }
 
+// Host spill, device side: outbox row i of beam b (Tp elements at outbox_off + i * Tp of the
+// beam's ring region, just filled from host memory) is appended to device ring row rows[i]
+// at ring time chunk * Tp, i.e. at positions (chunk * Tp + t) mod cap, t < Tp. The row's
+// capacity is at least Tp, so a copy wraps at most once. One workgroup per (row, beam).
+template<typename E>
+__global__ void __launch_bounds__(256)
+k_spill_scatter(E *ring, long bstride, long outbox_off, const int *rows, const long *row_off, const long *row_cap,
+                int Tp, long chunk, int beam0)
+{
+    const int i = blockIdx.x, b = blockIdx.y;
+    const long r = rows[i];
+    const long cap = row_cap[r];
+    const long pos0 = (chunk * long(Tp)) % cap;
+    E *base = ring + long(beam0 + b) * bstride;
+    const E *src = base + outbox_off + long(i) * Tp;
+    E *dst = base + row_off[r];
+    for (int t = threadIdx.x; t < Tp; t += 256) {
+        long pos = pos0 + t;
+        if (pos >= cap)
+            pos -= cap;
+        dst[pos] = src[t];
+    }
+}
+
+void launch_spill_scatter(void *ring, long bstride, int elt_bytes, long outbox_off, const int *rows, long nrows,
+                          const long *row_off, const long *row_cap, long Tp, long chunk, int beam0, int B, hipStream_t st)
+{
+    if (nrows == 0)
+        return;
+    const dim3 grid{unsigned(nrows), unsigned(B), 1u}, block{256u, 1u, 1u};
+    if (elt_bytes == 4)
+        hipLaunchKernelGGL(k_spill_scatter<uint32_t>, grid, block, 0, st, reinterpret_cast<uint32_t *>(ring), bstride,
+                           outbox_off, rows, row_off, row_cap, int(Tp), chunk, beam0);
+    else
+        hipLaunchKernelGGL(k_spill_scatter<uint16_t>, grid, block, 0, st, reinterpret_cast<uint16_t *>(ring), bstride,
+                           outbox_off, rows, row_off, row_cap, int(Tp), chunk, beam0);
+}
+
 void launch_ringbuf_copy(void *ringbuf, const uint32_t *octuples, long noct, long seg_bytes, int nbeams,
                          unsigned long long iframe, hipStream_t st)
 {

@@ -95,6 +95,10 @@ void launch_dequant(Dtype dt, const void *data, const void *scoff, void *out, lo
 
 
 // ---- ring-buffer segment copies driven by octuples (device array of 8*noct uint32) ----
+// Host spill: appends outbox row i (Tp elements of elt_bytes) of each beam of the batch to
+// device ring row rows[i] at ring time chunk * Tp.
+void launch_spill_scatter(void *ring, long bstride, int elt_bytes, long outbox_off, const int *rows, long nrows,
+                          const long *row_off, const long *row_cap, long Tp, long chunk, int beam0, int B, hipStream_t st);
 void launch_ringbuf_copy(void *ringbuf, const uint32_t *octuples, long noct, long seg_bytes, int nbeams,
                          unsigned long long iframe, hipStream_t st);
 

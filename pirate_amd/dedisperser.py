@@ -127,6 +127,12 @@ class GpuDedisperser:
     def allocate(self, gpu_allocator=None, host_allocator=None):
         L.check(L.lib().pdd_dedisperser_allocate(self._h))
 
+    def host_bytes(self):
+        """Pinned host memory of lag-ring rows spilled to the host (config max_gpu_clag)."""
+        n = ctypes.c_int64()
+        L.check(L.lib().pdd_dedisperser_host_bytes(self._h, ctypes.byref(n)))
+        return n.value
+
     def gpu_bytes(self):
         n = ctypes.c_int64()
         L.check(L.lib().pdd_dedisperser_gpu_bytes(self._h, ctypes.byref(n)))

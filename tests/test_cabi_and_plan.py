@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     for name in sorted(declared):
         assert hasattr(L, name), f'{name} declared in pirate_amd.h but not exported'
     assert declared == set(_lib.EXPORTED_SYMBOLS)
-    assert L.pdd_abi_version() == 2
+    assert L.pdd_abi_version() == 3
 
 
 def test_plain_c_caller(tmp_path):
