@@ -56,6 +56,8 @@ def kernel_alg_bytes(plan, dd, name):
         if name.startswith('stage2_t'):
             return B * (2**(t.amb_rank + t.dd_rank) + streams) * t.nt_ds * S
         return B * (streams * t.nt_ds * S + t.ndm_out * t.nt_out * (S + 4))
+    if name.startswith('spill_p'):
+        return 0    # host-spill scatter: its traffic is PCIe copies, not part of the HBM byte model
     raise ValueError(name)
 
 
@@ -134,7 +136,8 @@ def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
     dd = pa.GpuDedisperser(plan)
     dd.allocate()
     gib = dd.gpu_bytes() / 2**30
-    print(f'[bench] {plan.dtype}: allocated {gib:.2f} GiB of device memory', file=sys.stderr, flush=True)
+    print(f'[bench] {plan.dtype}: allocated {gib:.2f} GiB of device memory, {dd.host_bytes() / 2**30:.2f} GiB of '
+          f'pinned host memory', file=sys.stderr, flush=True)
 
     from oracle import oracle as orc   # weights helper only (bare-kernel weights)
     for it in range(plan.ntrees):

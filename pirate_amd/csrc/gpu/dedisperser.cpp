@@ -52,8 +52,10 @@ GpuDedisperser::GpuDedisperser(std::shared_ptr<DedispersionPlan> plan_, const Gp
 
 GpuDedisperser::~GpuDedisperser()
 {
-    if (compute_stream)
-        (void) hipStreamSynchronize(compute_stream);
+    // Copies of spilled ring rows may still be in flight on the copy streams.
+    for (hipStream_t x : {compute_stream, d2h_stream, h2d_stream})
+        if (x)
+            (void) hipStreamSynchronize(x);
     auto freeit = [](DeviceArray &a) { if (a.ptr) (void) hipFree(a.ptr); a.ptr = nullptr; };
     freeit(input_slots);
     freeit(grid_buf);
@@ -72,7 +74,11 @@ GpuDedisperser::~GpuDedisperser()
         timing_event_pool.push_back(t.start);
         timing_event_pool.push_back(t.stop);
     }
-    for (auto *v : {&ev_input_free, &ev_input_ready, &ev_output_ready, &ev_output_released, &timing_event_pool})
+    for (hipStream_t x : {d2h_stream, h2d_stream})
+        if (x)
+            (void) hipStreamDestroy(x);
+    for (auto *v : {&ev_input_free, &ev_input_ready, &ev_output_ready, &ev_output_released, &timing_event_pool,
+                    &ev_spill_s1, &ev_spill_d2h, &ev_spill_h2d, &ev_spill_scatter})
         for (hipEvent_t e : *v)
             (void) hipEventDestroy(e);
     if (compute_stream)
@@ -319,6 +325,7 @@ void GpuDedisperser::allocate()
     spill_k.assign(npri, {});
     nspill.assign(npri, 0);
     outbox_off.assign(npri, 0);
+    inbox_off.assign(npri, 0);
     host_bstride.assign(npri, 0);
     spill_rows.resize(npri);
     host_ring.assign(npri, nullptr);
@@ -357,6 +364,7 @@ void GpuDedisperser::allocate()
         std::stable_sort(order.begin(), order.end(), [&](long x, long y) { return kr[x] < kr[y]; });
         nspill[p] = long(order.size());
         outbox_off[p] = align_up(tot, 64);
+        inbox_off[p] = outbox_off[p] + nspill[p] * Tp;
         std::vector<long> off1(off), cap1(cap);
         std::vector<int> srows(order.size());
         long hoff = 0;
@@ -375,7 +383,7 @@ void GpuDedisperser::allocate()
         }
         host_bstride[p] = align_up(hoff, 64);
         spill_k[p] = kr;
-        ring_bstride[p] = align_up(outbox_off[p] + nspill[p] * Tp, 64);
+        ring_bstride[p] = align_up(inbox_off[p] + nspill[p] * Tp, 64);
         PDD_CHECK(ring_bstride[p] < (1L << 31), "GpuDedisperser: lag ring of primary tree " << p
                   << " exceeds 2^31 elements per beam (tree kernels use 32-bit row offsets)");
         rings[p] = dalloc(size_t(total_beams) * ring_bstride[p] * elt_bytes);
@@ -447,6 +455,14 @@ void GpuDedisperser::allocate()
         for (auto &e : v)
             HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     };
+    mkev(ev_spill_s1, npri * nbatches);
+    mkev(ev_spill_d2h, npri * nbatches);
+    mkev(ev_spill_h2d, npri * nbatches);
+    mkev(ev_spill_scatter, npri * nbatches);
+    if (std::any_of(nspill.begin(), nspill.end(), [](long n) { return n > 0; })) {
+        HIP_CHECK(hipStreamCreateWithFlags(&d2h_stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&h2d_stream, hipStreamNonBlocking));
+    }
     mkev(ev_input_free, nstreams);
     mkev(ev_input_ready, nstreams);
     mkev(ev_output_ready, nbatches_out);
@@ -495,6 +511,9 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         launch_grid_lds(dtype, g, st);
         timed_end("grid_lds", st, ev);
     }
+    for (long p = 0; p < npri; p++)
+        if (nspill[p] > 0)
+            spill_h2d(p, ichunk, ibatch, beam0);
     for (long p = 0; p < npri; p++) {
         Stage1Args s;
         const long Tp = nt_in >> p;
@@ -515,11 +534,16 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         hipEvent_t ev;
         const std::string nm = "stage1_p" + std::to_string(p);
         timed_begin(nm, st, ev);
+        if (nspill[p] > 0)     // the outbox is free once the previous chunk's copy has read it
+            HIP_CHECK(hipStreamWaitEvent(st, ev_spill_d2h[p * nbatches + ibatch], 0));
         launch_stage1_fast(dtype, s, st);
         timed_end(nm, st, ev);
         if (nspill[p] > 0)
-            spill_exchange(p, ichunk, beam0, st);
+            spill_d2h(p, ichunk, ibatch, beam0, st);
     }
+    for (long p = 0; p < npri; p++)
+        if (nspill[p] > 0)
+            spill_scatter(p, ichunk, ibatch, beam0, st);
     for (long it = 0; it < ntrees; it++) {
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
@@ -601,36 +625,66 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
 }
 
 
-// Host spill for primary tree p after stage 1 of (chunk, batch): outbox -> host slot
-// (chunk mod (k+1)), host slot ((chunk - k) mod (k+1)) -> outbox, outbox -> device ring rows
-// at ring time chunk * Tp. All on 'st', in this order, so the outbox is read before it is
-// refilled and stage 2 sees the rows appended. A slot not yet written holds zeros (the
-// stream before time 0).
-void GpuDedisperser::spill_exchange(long p, long ichunk, int beam0, hipStream_t st)
+// Host spill of primary tree p for (chunk, batch), on two copy streams so that both PCIe
+// directions run at once and overlap the kernels:
+//   spill_h2d   (issued first): host slots ((chunk - k) mod (k+1)) -> inbox, after the
+//               previous chunk's device-to-host copy of this batch (which wrote those
+//               slots, k >= 1) and its scatter (which read the inbox);
+//   spill_d2h   (after stage 1): outbox -> host slot (chunk mod (k+1));
+//   spill_scatter (on the compute stream, before stage 2): inbox -> device ring rows at
+//               ring time chunk * Tp.
+// Stage 1 waits for the previous chunk's device-to-host copy before it refills the outbox.
+// A slot not yet written holds zeros (the stream before time 0).
+void GpuDedisperser::spill_copy(long p, long ichunk, int beam0, bool to_host)
 {
     const long Tp = nt_in >> p, B = beams_per_batch;
     const size_t rb_pitch = size_t(ring_bstride[p]) * elt_bytes, hb_pitch = size_t(host_bstride[p]) * elt_bytes;
     char *ring_b = (char *) rings[p].ptr + size_t(beam0) * rb_pitch;
     char *host_b = (char *) host_ring[p] + size_t(beam0) * hb_pitch;
+    for (const SpillGroup &g : spill_groups[p]) {
+        const size_t width = size_t(g.nrows) * Tp * elt_bytes;
+        if (to_host) {
+            char *dev = ring_b + size_t(outbox_off[p] + g.row0 * Tp) * elt_bytes;
+            char *host = host_b + size_t(g.host_off + (ichunk % (g.k + 1)) * g.nrows * Tp) * elt_bytes;
+            HIP_CHECK(hipMemcpy2DAsync(host, hb_pitch, dev, rb_pitch, width, B, hipMemcpyDeviceToHost, d2h_stream));
+        } else {
+            const long slot = (((ichunk - g.k) % (g.k + 1)) + (g.k + 1)) % (g.k + 1);
+            char *dev = ring_b + size_t(inbox_off[p] + g.row0 * Tp) * elt_bytes;
+            char *host = host_b + size_t(g.host_off + slot * g.nrows * Tp) * elt_bytes;
+            HIP_CHECK(hipMemcpy2DAsync(dev, rb_pitch, host, hb_pitch, width, B, hipMemcpyHostToDevice, h2d_stream));
+        }
+    }
+}
+
+void GpuDedisperser::spill_h2d(long p, long ichunk, long ibatch, int beam0)
+{
+    const long e = p * nbatches + ibatch;
+    HIP_CHECK(hipStreamWaitEvent(h2d_stream, ev_spill_d2h[e], 0));
+    HIP_CHECK(hipStreamWaitEvent(h2d_stream, ev_spill_scatter[e], 0));
+    spill_copy(p, ichunk, beam0, false);
+    HIP_CHECK(hipEventRecord(ev_spill_h2d[e], h2d_stream));
+}
+
+void GpuDedisperser::spill_d2h(long p, long ichunk, long ibatch, int beam0, hipStream_t st)
+{
+    const long e = p * nbatches + ibatch;
+    HIP_CHECK(hipEventRecord(ev_spill_s1[e], st));
+    HIP_CHECK(hipStreamWaitEvent(d2h_stream, ev_spill_s1[e], 0));
+    spill_copy(p, ichunk, beam0, true);
+    HIP_CHECK(hipEventRecord(ev_spill_d2h[e], d2h_stream));
+}
+
+void GpuDedisperser::spill_scatter(long p, long ichunk, long ibatch, int beam0, hipStream_t st)
+{
+    const long e = p * nbatches + ibatch;
+    HIP_CHECK(hipStreamWaitEvent(st, ev_spill_h2d[e], 0));
     hipEvent_t ev;
     timed_begin("spill_p" + std::to_string(p), st, ev);
-    for (const SpillGroup &g : spill_groups[p]) {
-        const size_t width = size_t(g.nrows) * Tp * elt_bytes;
-        char *dev = ring_b + size_t(outbox_off[p] + g.row0 * Tp) * elt_bytes;
-        char *host = host_b + size_t(g.host_off + (ichunk % (g.k + 1)) * g.nrows * Tp) * elt_bytes;
-        HIP_CHECK(hipMemcpy2DAsync(host, hb_pitch, dev, rb_pitch, width, B, hipMemcpyDeviceToHost, st));
-    }
-    for (const SpillGroup &g : spill_groups[p]) {
-        const size_t width = size_t(g.nrows) * Tp * elt_bytes;
-        const long slot = (((ichunk - g.k) % (g.k + 1)) + (g.k + 1)) % (g.k + 1);
-        char *dev = ring_b + size_t(outbox_off[p] + g.row0 * Tp) * elt_bytes;
-        char *host = host_b + size_t(g.host_off + slot * g.nrows * Tp) * elt_bytes;
-        HIP_CHECK(hipMemcpy2DAsync(dev, rb_pitch, host, hb_pitch, width, B, hipMemcpyHostToDevice, st));
-    }
-    launch_spill_scatter(rings[p].ptr, ring_bstride[p], int(elt_bytes), outbox_off[p], (const int *) spill_rows[p].ptr,
-                         nspill[p], (const long *) ring_row_off[p].ptr, (const long *) ring_row_cap[p].ptr, Tp, ichunk,
-                         beam0, int(B), st);
+    launch_spill_scatter(rings[p].ptr, ring_bstride[p], int(elt_bytes), inbox_off[p], (const int *) spill_rows[p].ptr,
+                         nspill[p], (const long *) ring_row_off[p].ptr, (const long *) ring_row_cap[p].ptr, nt_in >> p,
+                         ichunk, beam0, int(beams_per_batch), st);
     timed_end("spill_p" + std::to_string(p), st, ev);
+    HIP_CHECK(hipEventRecord(ev_spill_scatter[e], st));
 }
 
 

@@ -126,13 +126,16 @@ private:
     // smallest stage-2 lag spans more than max_gpu_clag chunks is delayed by k_r whole chunks
     // in pinned host memory: stage 1 writes the row's chunk to an outbox at the end of the
     // beam's ring region, the outbox goes to host slot (chunk mod (k_r+1)), the slot written
-    // k_r chunks earlier comes back into the outbox, and a scatter kernel appends it to the
-    // row's (shorter) device ring, which stage 2 then reads with lag - k_r * Tp. Rows are
-    // grouped by k_r, so each group is one 2-D copy per direction per batch.
+    // k_r chunks earlier comes back into an inbox beside it, and a scatter kernel appends it
+    // to the row's (shorter) device ring, which stage 2 then reads with lag - k_r * Tp. Rows
+    // are grouped by k_r, so each group is one 2-D copy per direction per batch. The two
+    // directions run on their own streams, ordered by events per (primary tree, batch).
     struct SpillGroup { long k, row0, nrows, host_off; };
     std::vector<std::vector<SpillGroup>> spill_groups;       // [p]
     std::vector<std::vector<long>> spill_k;                  // [p][ring row] host delay in chunks
-    std::vector<long> nspill, outbox_off, host_bstride;      // [p]
+    std::vector<long> nspill, outbox_off, inbox_off, host_bstride;   // [p]
+    hipStream_t d2h_stream = nullptr, h2d_stream = nullptr;
+    std::vector<hipEvent_t> ev_spill_s1, ev_spill_d2h, ev_spill_h2d, ev_spill_scatter;   // [p * nbatches + batch]
     std::vector<DeviceArray> spill_rows;                     // [p] device int: ring row of each outbox row
     std::vector<DeviceArray> ring1_row_off, ring1_row_cap;   // [p] stage-1 (write) side
     std::vector<void *> host_ring;                           // [p] pinned, (total_beams, host_bstride)
@@ -166,7 +169,10 @@ private:
 
     DeviceArray dalloc(size_t nbytes);
     void check_stopped_locked();
-    void spill_exchange(long p, long ichunk, int beam0, hipStream_t st);
+    void spill_copy(long p, long ichunk, int beam0, bool to_host);
+    void spill_h2d(long p, long ichunk, long ibatch, int beam0);
+    void spill_d2h(long p, long ichunk, long ibatch, int beam0, hipStream_t st);
+    void spill_scatter(long p, long ichunk, long ibatch, int beam0, hipStream_t st);
     void build_tables();
     void validate_gpu_limits();
 };
