@@ -174,6 +174,24 @@ int pdd_fill_analytic_weights(pdd_dedisperser *dd, const double *freq_variances,
 /* GpuDedisperser::stop(e), GpuDedisperser.cpp:520-577. */
 int pdd_stop(pdd_dedisperser *dd, const char *reason);
 
+/* ---- output sharing with a consumer process ---- */
+
+/* Output-ring export (the reference's IPC handshake of FrbServer::_fill_handshake,
+ * src_lib/FrbServer.cpp:1583-1612): handle (64 bytes, a hipIpcMemHandle_t) of the one device
+ * allocation that holds every tree's out_max and out_argmax ring, and its size. Each array
+ * is (nbatches_out * beams_per_batch, ndm_out, nt_out) of the config dtype (out_max) or
+ * uint32 (out_argmax), contiguous, at the byte offset pdd_output_array_offset gives
+ * (which: 0 = out_max, 1 = out_argmax); output seq_id lives in slot seq_id % nbatches_out.
+ * The producer and consumer agree on released seq_ids out of band, as the reference's
+ * grouper protocol does. Regions of 2 GiB or more are refused. */
+int pdd_output_ipc_handle(pdd_dedisperser *dd, void *handle, int64_t *nbytes);
+int pdd_output_array_offset(const pdd_dedisperser *dd, int64_t itree, int which, int64_t *byte_offset);
+
+/* Consumer side (FrbGrouper.cpp:565-581): maps a handle from another process of this node
+ * on HIP device 'device'; *base receives its device address. pdd_ipc_close unmaps it. */
+int pdd_ipc_open(const void *handle, int device, void **base);
+int pdd_ipc_close(void *base, int device);
+
 /* ---- input stage ---- */
 
 /* GpuDequantizationKernel::launch (include/pirate/GpuDequantizationKernel.hpp:56-65; semantics of

@@ -6,11 +6,11 @@ over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library wit
 """
 
 from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
-from .dedisperser import GpuDedisperser, GpuDedisperserOutputs
+from .dedisperser import GpuDedisperser, GpuDedisperserOutputs, OutputRingView
 from .kernels import GpuDequantizationKernel, GpuRingbufCopyKernel
 from .pf_variance import PfAvarApproximation
 from .offline import AssembledFrame, OfflineDedisperser, XEngineMetadata
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
-           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
+           'GpuDedisperser', 'GpuDedisperserOutputs', 'OutputRingView', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
            'PfAvarApproximation', 'OfflineDedisperser', 'AssembledFrame', 'XEngineMetadata']

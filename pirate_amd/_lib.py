@@ -58,6 +58,10 @@ _SIGS = {
     'pdd_dedisperser_allocate': (ctypes.c_int, [vp]),
     'pdd_dedisperser_gpu_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
     'pdd_dedisperser_host_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+    'pdd_output_ipc_handle': (ctypes.c_int, [vp, vp, ctypes.POINTER(i64)]),
+    'pdd_output_array_offset': (ctypes.c_int, [vp, i64, ctypes.c_int, ctypes.POINTER(i64)]),
+    'pdd_ipc_open': (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp)]),
+    'pdd_ipc_close': (ctypes.c_int, [vp, ctypes.c_int]),
     'pdd_acquire_input': (ctypes.c_int, [vp, i64, vp, ctypes.POINTER(vp)]),
     'pdd_release_input_and_launch': (ctypes.c_int, [vp, i64, vp]),
     'pdd_acquire_output': (ctypes.c_int, [vp, i64, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),

@@ -45,6 +45,12 @@ static int fail(const std::string &msg)
     catch (std::exception &e) { (dd)->d->stop(e.what()); return fail(e.what()); } \
     catch (...) { (dd)->d->stop("unknown C++ exception"); return fail("unknown C++ exception"); }
 
+static void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        throw pdd_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+}
+
 static int copy_string(const std::string &s, char *buf, size_t buflen, size_t *needed)
 {
     if (needed)
@@ -267,6 +273,44 @@ int pdd_dedisperser_host_bytes(const pdd_dedisperser *dd, int64_t *nbytes)
     PDD_TRY({ PDD_CHECK(dd && nbytes, "null argument"); *nbytes = int64_t(dd->d->host_bytes_allocated); });
 }
 
+int pdd_output_ipc_handle(pdd_dedisperser *dd, void *handle, int64_t *nbytes)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && handle && nbytes, "null argument");
+        size_t n = 0;
+        dd->d->output_ipc_handle(handle, &n);
+        *nbytes = int64_t(n);
+    });
+}
+
+int pdd_output_array_offset(const pdd_dedisperser *dd, int64_t itree, int which, int64_t *byte_offset)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && byte_offset, "null argument");
+        *byte_offset = int64_t(dd->d->output_array_offset(itree, which));
+    });
+}
+
+int pdd_ipc_open(const void *handle, int device, void **base)
+{
+    PDD_TRY({
+        PDD_CHECK(handle && base, "null argument");
+        hipIpcMemHandle_t h;
+        std::memcpy(&h, handle, sizeof(h));
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipIpcOpenMemHandle(base, h, hipIpcMemLazyEnablePeerAccess), "hipIpcOpenMemHandle");
+    });
+}
+
+int pdd_ipc_close(void *base, int device)
+{
+    PDD_TRY({
+        PDD_CHECK(base, "null argument");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipIpcCloseMemHandle(base), "hipIpcCloseMemHandle");
+    });
+}
+
 int pdd_acquire_input(pdd_dedisperser *dd, int64_t seq_id, void *stream, void **slot)
 {
     PDD_TRY_DD(dd, {
@@ -373,11 +417,6 @@ int pdd_dequantize(const char *out_dtype, int64_t nbeams, int64_t nfreq, int64_t
     });
 }
 
-static void hip_check(hipError_t e, const char *what)
-{
-    if (e != hipSuccess)
-        throw pdd_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
-}
 
 struct pdd_ringbuf_copy {
     long total_beams = 0, beams_per_batch = 0, seg_bytes = 0, noct = 0;

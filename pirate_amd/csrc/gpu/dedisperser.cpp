@@ -67,9 +67,10 @@ GpuDedisperser::~GpuDedisperser()
             (void) hipHostFree(h);
     for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &ring1_row_off, &ring1_row_cap, &spill_rows,
                     &stage2_lags, &sb, &d_main_emit, &d_stag,
-                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_max, &out_argmax, &out_var})
+                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_var})
         for (auto &a : *v)
             freeit(a);
+    freeit(out_region);     // out_max / out_argmax are views into it
     for (auto &t : timed_pending) {
         timing_event_pool.push_back(t.start);
         timing_event_pool.push_back(t.stop);
@@ -444,10 +445,31 @@ void GpuDedisperser::allocate()
                                                           : stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts));
         s2_state[it] = dalloc(size_t(total_beams) * s2_state_bstride[it] * sizeof(float));
         weights[it] = dalloc(size_t(nbatches_wt) * B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N * sizeof(float));
-        out_max[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * elt_bytes);
-        out_argmax[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * t.nt_out * sizeof(unsigned int));
+        // out_max / out_argmax: views into the output region (below)
+        out_max[it].nbytes = size_t(nbatches_out) * B * t.ndm_out * t.nt_out * elt_bytes;
+        out_argmax[it].nbytes = size_t(nbatches_out) * B * t.ndm_out * t.nt_out * sizeof(unsigned int);
         if (params.enable_variances)
             out_var[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * T.M * t.nprofiles * sizeof(double));
+    }
+
+    // Output ring: every tree's out_max and out_argmax in one device allocation (256-byte
+    // aligned sub-arrays, tree-major), so that one IPC handle exports all of it to a
+    // consumer process (the reference shares its output ring the same way:
+    // FrbServer.cpp:1583-1612, FrbGrouper.cpp:565-581).
+    {
+        size_t tot = 0;
+        out_offsets.assign(2 * ntrees, 0);
+        for (long it = 0; it < ntrees; it++) {
+            out_offsets[2 * it] = tot;
+            tot = size_t(align_up(long(tot + out_max[it].nbytes), 256));
+            out_offsets[2 * it + 1] = tot;
+            tot = size_t(align_up(long(tot + out_argmax[it].nbytes), 256));
+        }
+        out_region = dalloc(tot);
+        for (long it = 0; it < ntrees; it++) {
+            out_max[it].ptr = (char *) out_region.ptr + out_offsets[2 * it];
+            out_argmax[it].ptr = (char *) out_region.ptr + out_offsets[2 * it + 1];
+        }
     }
 
     auto mkev = [](std::vector<hipEvent_t> &v, long n) {
@@ -978,6 +1000,27 @@ std::string GpuDedisperser::kernel_timing_report()
     for (auto &kv : timing_totals)
         os << kv.first << " " << kv.second.first << " " << kv.second.second << "\n";
     return os.str();
+}
+
+
+void GpuDedisperser::output_ipc_handle(void *handle, size_t *nbytes)
+{
+    PDD_CHECK(is_allocated, "output_ipc_handle(): allocate() has not been called");
+    PDD_CHECK(out_region.nbytes < (size_t(1) << 31),
+              "output_ipc_handle(): the output ring is " << out_region.nbytes << " bytes; this build exports at most 2 GiB");
+    static_assert(sizeof(hipIpcMemHandle_t) == 64, "IPC handle size");
+    HIP_CHECK(hipSetDevice(params.device));
+    hipIpcMemHandle_t h;
+    HIP_CHECK(hipIpcGetMemHandle(&h, out_region.ptr));
+    std::memcpy(handle, &h, sizeof(h));
+    *nbytes = out_region.nbytes;
+}
+
+size_t GpuDedisperser::output_array_offset(long itree, int which) const
+{
+    PDD_CHECK(is_allocated, "output_array_offset(): allocate() has not been called");
+    PDD_CHECK(itree >= 0 && itree < ntrees && (which == 0 || which == 1), "output_array_offset(): bad tree or array");
+    return out_offsets[2 * itree + which];
 }
 
 

@@ -86,6 +86,13 @@ public:
     // and the outputs go to output slot 'oslot'. Used by the timing harness.
     void launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t stream);
 
+    // Output ring export to a consumer process: a 64-byte hipIpcMemHandle_t of the device
+    // allocation that holds every tree's out_max and out_argmax ring (nbatches_out slots of
+    // (B, ndm_out, nt_out)), its size, and each array's byte offset in it (which: 0 =
+    // out_max, 1 = out_argmax).
+    void output_ipc_handle(void *handle, size_t *nbytes);
+    size_t output_array_offset(long itree, int which) const;
+
     // Copies an internal buffer to the host (tests): name in {"grid", "lds<p>", "sb<t>"}.
     size_t debug_copy(const std::string &name, void *host, size_t maxbytes);
 
@@ -146,7 +153,9 @@ private:
     std::vector<DeviceArray> s1_state; std::vector<long> s1_state_bstride;
     std::vector<DeviceArray> s2_state; std::vector<long> s2_state_bstride;
     std::vector<DeviceArray> weights; // [itree] (nbatches_wt, B, ndm_wt, nt_wt, P, N) float
-    std::vector<DeviceArray> out_max, out_argmax;   // [itree] (nbatches_out, B, ndm_out, nt_out)
+    std::vector<DeviceArray> out_max, out_argmax;   // [itree] (nbatches_out, B, ndm_out, nt_out), views into out_region
+    DeviceArray out_region;
+    std::vector<size_t> out_offsets;                // [2 * itree + which]
     std::vector<DeviceArray> out_var; // [itree] (nbatches_out, B, ndm_out, M, P) float64, if enabled
     DeviceArray grid_rows;            // int (3*nchan) f0, cnt, woff
     DeviceArray grid_w;               // float

@@ -127,6 +127,28 @@ class GpuDedisperser:
     def allocate(self, gpu_allocator=None, host_allocator=None):
         L.check(L.lib().pdd_dedisperser_allocate(self._h))
 
+    def output_ipc_handle(self):
+        """(handle, layout) exporting the output ring to a consumer process of this node:
+        handle is 64 bytes (hipIpcMemHandle_t) of the device allocation holding every tree's
+        out_max and out_argmax ring; layout is a picklable dict (region size, dtype, slot
+        count, per-tree byte offsets and shapes) for OutputRingView. Output seq_id lives in
+        slot seq_id % nbatches_out; which seq_ids are ready is agreed out of band."""
+        h = ctypes.create_string_buffer(64)
+        n = ctypes.c_int64()
+        L.check(L.lib().pdd_output_ipc_handle(self._h, h, ctypes.byref(n)))
+        trees = []
+        for it, t in enumerate(self.trees):
+            offs = []
+            for which in (0, 1):
+                o = ctypes.c_int64()
+                L.check(L.lib().pdd_output_array_offset(self._h, it, which, ctypes.byref(o)))
+                offs.append(o.value)
+            trees.append(dict(max_offset=offs[0], argmax_offset=offs[1],
+                              shape=(self.nbatches_out * self.beams_per_batch, t.ndm_out, t.nt_out)))
+        layout = dict(nbytes=n.value, dtype=self.dtype, nbatches_out=self.nbatches_out,
+                      beams_per_batch=self.beams_per_batch, device=self.cuda_device_id, trees=trees)
+        return h.raw, layout
+
     def host_bytes(self):
         """Pinned host memory of lag-ring rows spilled to the host (config max_gpu_clag)."""
         n = ctypes.c_int64()
@@ -272,3 +294,41 @@ class GpuDedisperser:
         buf = np.zeros(n.value, dtype=np.uint8)
         L.check(L.lib().pdd_debug_copy(self._h, name.encode(), buf.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)))
         return buf
+
+
+class OutputRingView:
+    """Consumer-side view of a dedisperser's output ring exported by
+    GpuDedisperser.output_ipc_handle() in another process of the node (the reference's
+    grouper side, src_lib/FrbGrouper.cpp:565-599). out_max(itree, slot) and
+    out_argmax(itree, slot) are (beams_per_batch, ndm_out, nt_out) tensors on the GPU
+    (argmax tokens as int32 with the uint32 bits), valid until close()."""
+
+    def __init__(self, handle, layout):
+        self.layout = layout
+        self.device = int(layout['device'])
+        self._base = ctypes.c_void_p()
+        L.check(L.lib().pdd_ipc_open(bytes(handle), self.device, ctypes.byref(self._base)))
+
+    def _view(self, itree, slot, which):
+        if self._base is None or not self._base.value:
+            raise RuntimeError('OutputRingView: closed')
+        t = self.layout['trees'][itree]
+        if not 0 <= slot < self.layout['nbatches_out']:
+            raise IndexError(f'OutputRingView: slot {slot} out of range')
+        B = self.layout['beams_per_batch']
+        nslot, ndm, nt = t['shape']
+        dtype = self.layout['dtype'] if which == 0 else 'uint32'
+        esize = 2 if dtype in ('float16', 'bfloat16') else 4
+        off = t['max_offset' if which == 0 else 'argmax_offset'] + slot * B * ndm * nt * esize
+        return device_view(self._base.value + off, (B, ndm, nt), dtype)
+
+    def out_max(self, itree, slot):
+        return self._view(itree, slot, 0)
+
+    def out_argmax(self, itree, slot):
+        return self._view(itree, slot, 1)
+
+    def close(self):
+        if self._base is not None and self._base.value:
+            L.check(L.lib().pdd_ipc_close(self._base, self.device))
+        self._base = None
