@@ -71,6 +71,7 @@ GpuDedisperser::~GpuDedisperser()
         for (auto &a : *v)
             freeit(a);
     freeit(out_region);     // out_max / out_argmax are views into it
+    freeit(split_x);
     for (auto &t : timed_pending) {
         timing_event_pool.push_back(t.start);
         timing_event_pool.push_back(t.stop);
@@ -183,6 +184,7 @@ void GpuDedisperser::build_tables()
             else
                 T.out_stream[row] = s0;
         }
+        T.split = !T.fast && dd == 8;
 
         for (int pf_dm = 0; pf_dm < pf_ndm; pf_dm++)
             for (int m = 0; m < M; m++) {
@@ -222,18 +224,14 @@ void GpuDedisperser::validate_gpu_limits()
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
         PDD_CHECK(t.dd_rank <= kGpuMaxDD, "GpuDedisperser: stage-2 dd_rank " << t.dd_rank << " not supported by this build (max " << kGpuMaxDD << ")");
-        if (!T.fast) {
-            PDD_CHECK(t.dd_rank <= kGpuMaxDDGeneral, "GpuDedisperser: tree " << it << ": stage-2 dd_rank " << t.dd_rank
-                      << " with this subband search (streams before the last pass) is not supported by this build (max "
-                      << kGpuMaxDDGeneral << "; rank 8 needs subband counts [0,...,0,1])");
+        if (!T.fast && t.dd_rank <= kGpuMaxDDGeneral)
             PDD_CHECK(stage2_lds_bytes(t.dd_rank, T.stag_hist_nelts) <= kLdsBytesPerCU,
                       "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
+        if (!T.fast && t.dd_rank == 8) {
+            PDD_CHECK(t.frequency_subbands.pf_rank <= 4, "GpuDedisperser: tree " << it << ": rank-8 split needs pf_rank <= 4");
+            PDD_CHECK(split_b_lds_bytes(T.stag_hist_nelts) <= kLdsBytesPerCU,
+                      "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
         }
-        std::vector<int> per_pass(t.dd_rank, 0);
-        for (int k = 0; k < T.nstag; k++)
-            per_pass[T.stag[6 * k]]++;
-        for (int c : per_pass)
-            PDD_CHECK(c <= 16, "GpuDedisperser: tree " << it << " has more than 16 staggered subband pairs in one pass (not supported by this build)");
         PDD_CHECK(t.pf.time_downsampling <= 512, "GpuDedisperser: time_downsampling too large");
         PDD_CHECK(pf_config_supported(T.M, T.tpad, T.nlev, int(t.Dcore), int(t.pf.time_downsampling), t.nt_ds),
                   "GpuDedisperser: tree " << it << ": peak-finder shape not supported (M=" << T.M << ")");
@@ -403,6 +401,7 @@ void GpuDedisperser::allocate()
         s1_state[p] = dalloc(size_t(total_beams) * s1_state_bstride[p] * sizeof(float));
     }
 
+    size_t split_nelts = 0;   // float32 scratch of rank-8 split stage-2 trees
     stage2_lags.resize(ntrees);
     sb.resize(ntrees);
     d_main_emit.resize(ntrees);
@@ -442,7 +441,10 @@ void GpuDedisperser::allocate()
         d_stag_begin[it] = up_int(T.stag_begin);
         d_out_stream[it] = up_int(T.out_stream);
         s2_state_bstride[it] = pow2(t.amb_rank) * (T.fast ? fast_tree_state_nelts(t.dd_rank)
-                                                          : stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts));
+                                                   : T.split ? split_state_nelts_per_tree(T.stag_hist_nelts)
+                                                             : stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts));
+        if (T.split)
+            split_nelts = std::max(split_nelts, size_t(B) * pow2(t.amb_rank) * 256 * t.nt_ds);
         s2_state[it] = dalloc(size_t(total_beams) * s2_state_bstride[it] * sizeof(float));
         weights[it] = dalloc(size_t(nbatches_wt) * B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N * sizeof(float));
         // out_max / out_argmax: views into the output region (below)
@@ -451,6 +453,9 @@ void GpuDedisperser::allocate()
         if (params.enable_variances)
             out_var[it] = dalloc(size_t(nbatches_out) * B * t.ndm_out * T.M * t.nprofiles * sizeof(double));
     }
+
+    if (split_nelts > 0)
+        split_x = dalloc(split_nelts * sizeof(float));
 
     // Output ring: every tree's out_max and out_argmax in one device allocation (256-byte
     // aligned sub-arrays, tree-major), so that one IPC handle exports all of it to a
@@ -605,6 +610,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         timed_begin(nm2, st, ev);
         if (T.fast)
             launch_stage2_fast(dtype, s, st);
+        else if (T.split)
+            launch_stage2_split(dtype, s, (float *) split_x.ptr, st);
         else
             launch_stage2(dtype, s, st);
         timed_end(nm2, st, ev);

@@ -109,6 +109,7 @@ public:
         std::vector<int> main_emit, stag, lvl0, fl, m_to_n, stag_begin;
         std::vector<int> out_stream;   // register-tree path: final row -> stream (-1: none)
         bool fast = false;             // stage 2 runs on the register-tree kernel
+        bool split = false;            // rank-8 subband search: two-launch split kernel
         int nstag = 0, nlvl0 = 0, stag_hist_nelts = 0, flmax = 0, tpad = 0, nlev = 0, rowlen = 0, M = 0, N = 0;
     };
     std::vector<TreeTables> tables;
@@ -155,6 +156,7 @@ private:
     std::vector<DeviceArray> weights; // [itree] (nbatches_wt, B, ndm_wt, nt_wt, P, N) float
     std::vector<DeviceArray> out_max, out_argmax;   // [itree] (nbatches_out, B, ndm_out, nt_out), views into out_region
     DeviceArray out_region;
+    DeviceArray split_x;              // float32 scratch of the rank-8 split stage 2
     std::vector<size_t> out_offsets;                // [2 * itree + which]
     std::vector<DeviceArray> out_var; // [itree] (nbatches_out, B, ndm_out, M, P) float64, if enabled
     DeviceArray grid_rows;            // int (3*nchan) f0, cnt, woff

@@ -125,10 +125,10 @@ struct Stage2Sink {
     const int *stag_begin;
     float *tile, *shist;
 
-    // Staggered outputs computed in the read phase, written in the write phase.
-    // At most 16 staggered pairs per pass (host-checked): 16*64/256 = 4 items per thread.
+    // Staggered butterflies of pass r, run before the pass (they read its input rows, which
+    // the pass overwrites only in its write phase) in rounds of SMAX entries per thread:
+    // each round reads, waits for every thread, then emits and updates the histories.
     static constexpr int SMAX = 4;
-    float so0[SMAX], so1[SMAX], ssv[SMAX];
 
     __device__ __forceinline__ void emit(int s, int t, float v)
     {
@@ -139,44 +139,46 @@ struct Stage2Sink {
         st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
     }
 
-    __device__ __forceinline__ void read_phase_extra(int r)
-    {
-        const int half = 1 << r;
-        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
-#pragma unroll
-        for (int q = 0; q < SMAX; q++) {
-            const int idx = q * kTreeThreads + threadIdx.x;
-            if (idx < n) {
-                const int k = k0 + idx / kTT, t = idx % kTT;
-                const int *e = stag + 6 * k;
-                const int row0 = e[1], row1 = e[2], L = e[3];
-                const float *h = shist + e[5];
-                const float xo = tile[row1 * kTT + t];
-                const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
-                const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
-                so0[q] = kRsqrt2 * (xa + xo);
-                so1[q] = kRsqrt2 * (xb + xo);
-                const int ts = t - (kTT - nv);
-                ssv[q] = (t < kTT - half) ? 0.0f : (ts >= 0) ? tile[row0 * kTT + ts] : h[half + ts];
-            }
-        }
-    }
+    __device__ void read_phase_extra(int) { }
+    __device__ void write_phase_extra(int) { }
 
-    __device__ __forceinline__ void write_phase_extra(int r)
+    __device__ void stag_rounds(int r)
     {
         const int half = 1 << r;
         const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+        for (int base = 0; base < n; base += SMAX * kTreeThreads) {
+            float so0[SMAX], so1[SMAX], ssv[SMAX];
 #pragma unroll
-        for (int q = 0; q < SMAX; q++) {
-            const int idx = q * kTreeThreads + threadIdx.x;
-            if (idx < n) {
-                const int k = k0 + idx / kTT, t = idx % kTT;
-                const int *e = stag + 6 * k;
-                emit(e[4], t, so0[q]);
-                emit(e[4] + 1, t, so1[q]);
-                if (t >= kTT - half)
-                    shist[e[5] + (t - (kTT - half))] = ssv[q];
+            for (int q = 0; q < SMAX; q++) {
+                const int idx = base + q * kTreeThreads + threadIdx.x;
+                if (idx < n) {
+                    const int k = k0 + idx / kTT, t = idx % kTT;
+                    const int *e = stag + 6 * k;
+                    const int row0 = e[1], row1 = e[2], L = e[3];
+                    const float *h = shist + e[5];
+                    const float xo = tile[row1 * kTT + t];
+                    const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
+                    const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
+                    so0[q] = kRsqrt2 * (xa + xo);
+                    so1[q] = kRsqrt2 * (xb + xo);
+                    const int ts = t - (kTT - nv);
+                    ssv[q] = (t < kTT - half) ? 0.0f : (ts >= 0) ? tile[row0 * kTT + ts] : h[half + ts];
+                }
             }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < SMAX; q++) {
+                const int idx = base + q * kTreeThreads + threadIdx.x;
+                if (idx < n) {
+                    const int k = k0 + idx / kTT, t = idx % kTT;
+                    const int *e = stag + 6 * k;
+                    emit(e[4], t, so0[q]);
+                    emit(e[4] + 1, t, so1[q]);
+                    if (t >= kTT - half)
+                        shist[e[5] + (t - (kTT - half))] = ssv[q];
+                }
+            }
+            __syncthreads();
         }
     }
 };
@@ -242,6 +244,7 @@ k_stage2(Stage2Args a)
                     sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
                 }
             }
+            sink.stag_rounds(r);
             tree_pass<DD>(r, tile, hist, a.main_emit, sink, nv);
         }
         if (r_lvl0 == DD) {
@@ -255,6 +258,332 @@ k_stage2(Stage2Args a)
 
     for (int i = tid; i < nst; i += kTreeThreads)
         st[i] = hist[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// Rank-8 stage 2 with subband searches, in two launches
+//
+// At rank 8 the LDS tile of k_stage2 would need 256 x 64 floats plus 128 x 255 history floats
+// (191 KB). The tree is split by rows instead: passes 0..3 pair rows inside groups of 16
+// consecutive rows (h = row >> 4), passes 4..7 pair rows with equal low 4 bits (j = row & 15),
+// and every subband stream comes from passes >= dd - pf_rank >= 4. Part A runs passes 0..3 per
+// (tree instance, h) on rows read from the lag ring and writes the tree after pass 3 to a
+// float32 scratch X (B, 2^amb, 256, Tp) of the current chunk; part B runs passes 4..7 per
+// (tree instance, j) on X with the full tree's lags, pair histories, staggered butterflies and
+// subband emissions. Every operation is the direct tree's, in its order: bit-identical.
+// A history longer than a tile (2^7 = 128 samples at pass 7) is refilled from the tile and
+// from its own older part.
+
+static constexpr int kSplitNPL = 8;                       // pairs per pass in a 16-row group
+static constexpr int kSplitHistA = kSplitNPL * 15;        // part A pair history (passes 0..3)
+static constexpr int kSplitHistB = kSplitNPL * (256 - 16);   // part B pair history (passes 4..7)
+
+__host__ __device__ static inline long split_nst(int stag_hist_nelts)
+{
+    return 16L * kSplitHistA + 16L * (kSplitHistB + stag_hist_nelts);
+}
+
+long split_state_nelts_per_tree(int stag_hist_nelts) { return split_nst(stag_hist_nelts); }
+
+struct NoSink {
+    __device__ void read_phase_extra(int) { }
+    __device__ void write_phase_extra(int) { }
+    __device__ void emit(int, int, float) { }
+};
+
+int split_b_lds_bytes(int stag_hist_nelts)
+{
+    return int(sizeof(float) * (16 * kTT + kSplitHistB + stag_hist_nelts));
+}
+
+template<typename Tst>
+__global__ void __launch_bounds__(kTreeThreads)
+k_stage2_split_a(Stage2Args a, float *X)
+{
+    using TD = TreeDims<4>;
+    __shared__ float tile[16 * kTT];
+    __shared__ float hist[TD::HIST];
+    const int tr = blockIdx.x >> 4, h = blockIdx.x & 15, b = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const long gbeam = long(a.beam0) + b;
+    float *st = a.state + gbeam * a.state_bstride + long(tr) * split_nst(a.stag_hist_nelts)
+                + long(h) * kSplitHistA;
+    for (int i = tid; i < TD::HIST; i += kTreeThreads)
+        hist[i] = st[i];
+    const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + gbeam * a.ring.bstride;
+    float *x = X + ((long(b) << a.amb) + tr) * (256L * a.Tp);
+    NoSink sink;
+    __syncthreads();
+
+    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
+        const int nv = min(kTT, a.Tp - t0);
+        const long tau0 = a.chunk * a.Tp + t0;
+        for (int l = __builtin_amdgcn_readfirstlane(tid >> 6); l < 16; l += kTreeThreads / 64) {
+            const long rr = (long(h * 16 + l) << a.amb) + tr;
+            const long cap = a.ring.row_cap[rr];
+            long pos = ring_pos(tau0 - a.row_lag[rr], cap) + lane;
+            if (pos >= cap)
+                pos -= cap;
+            tile[l * kTT + lane] = ld_st(ring + a.ring.row_off[rr] + pos);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int r = 0; r < 4; r++)
+            tree_pass<4>(r, tile, hist, nullptr, sink, nv);
+        for (int idx = tid; idx < 16 * kTT; idx += kTreeThreads) {
+            const int l = idx / kTT, t = idx % kTT;
+            if (t < nv)
+                x[long(h * 16 + l) * a.Tp + t0 + t] = tile[idx];
+        }
+        __syncthreads();
+    }
+
+    for (int i = tid; i < TD::HIST; i += kTreeThreads)
+        st[i] = hist[i];
+}
+
+// New content of history slot k (time nv - half + k relative to this tile) of a row whose
+// tile row is 'row' and whose old history is 'old' (time -half + k' at slot k').
+__device__ __forceinline__ float hist_refill(const float *tile, int row, const float *old, int half, int nv, int k)
+{
+    const int tm = nv - half + k;
+    return (tm >= 0) ? tile[row * kTT + tm] : old[nv + k];
+}
+
+// Part-B emissions: subband streams into the peak-finder rows, with staggered butterflies
+// and level-0 copies restricted to the group's rows (global row & 15 == g).
+template<typename Tst>
+struct SplitSink {
+    Tst *sbb;
+    const int *stag, *fl, *stag_begin;
+    int M, rowlen, tpad, dpf_base, t0, nv, g;
+    float *tile, *shist;
+    static constexpr int SMAX = 4, HB = 2;     // entries per thread per round; 64-slot history blocks
+
+    __device__ __forceinline__ void emit(int s, int t, float v)
+    {
+        if (t >= nv)
+            return;
+        const int pf_dm = s / M, m = s - pf_dm * M;
+        const long row = long(dpf_base + pf_dm) * M + m;
+        st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
+    }
+
+    __device__ void read_phase_extra(int) { }
+    __device__ void write_phase_extra(int) { }
+
+    // Staggered butterflies of pass r whose rows are in group g, before the pass (as in
+    // Stage2Sink::stag_rounds).
+    __device__ void stag_rounds(int r)
+    {
+        const int half = 1 << r;
+        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+        for (int base = 0; base < n; base += SMAX * kTreeThreads) {
+            float so0[SMAX], so1[SMAX], ssv[SMAX][HB];
+#pragma unroll
+            for (int q = 0; q < SMAX; q++) {
+                const int idx = base + q * kTreeThreads + threadIdx.x;
+                if (idx < n) {
+                    const int k = k0 + idx / kTT, t = idx % kTT;
+                    const int *e = stag + 6 * k;
+                    if ((e[1] & 15) == g) {
+                        const int row0 = e[1] >> 4, row1 = e[2] >> 4, L = e[3];
+                        const float *h = shist + e[5];
+                        const float xo = tile[row1 * kTT + t];
+                        const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
+                        const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
+                        so0[q] = kRsqrt2 * (xa + xo);
+                        so1[q] = kRsqrt2 * (xb + xo);
+                        if (half <= kTT) {
+                            if (t >= kTT - half)
+                                ssv[q][0] = hist_refill(tile, row0, h, half, nv, t - (kTT - half));
+                        } else {
+#pragma unroll
+                            for (int m = 0; m < HB; m++)
+                                if (m * kTT + t < half)
+                                    ssv[q][m] = hist_refill(tile, row0, h, half, nv, m * kTT + t);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < SMAX; q++) {
+                const int idx = base + q * kTreeThreads + threadIdx.x;
+                if (idx < n) {
+                    const int k = k0 + idx / kTT, t = idx % kTT;
+                    const int *e = stag + 6 * k;
+                    if ((e[1] & 15) == g) {
+                        emit(e[4], t, so0[q]);
+                        emit(e[4] + 1, t, so1[q]);
+                        float *h = shist + e[5];
+                        if (half <= kTT) {
+                            if (t >= kTT - half)
+                                h[t - (kTT - half)] = ssv[q][0];
+                        } else {
+#pragma unroll
+                            for (int m = 0; m < HB; m++)
+                                if (m * kTT + t < half)
+                                    h[m * kTT + t] = ssv[q][m];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+};
+
+// Pass r (4..7) of the rank-8 tree on the 16 rows of group g (tile row l = global row >> 4).
+template<typename Sink>
+__device__ __forceinline__ void split_pass(int r, int g, float *tile, float *hist, const int *emit, Sink &sink, int nv)
+{
+    constexpr int NPL = kSplitNPL, ITEMS = NPL * kTT / kTreeThreads, HB = 2;
+    const int tid = threadIdx.x;
+    const int rl = r - 4, hl = 1 << rl, half = 1 << r;
+    float *h = hist + NPL * (half - 16);
+
+    float o0[ITEMS], o1[ITEMS], sv[ITEMS][HB];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int idx = i * kTreeThreads + tid;
+        const int q = idx / kTT, t = idx % kTT;
+        const int el = ((q >> rl) << (rl + 1)) + (q & (hl - 1)), ol = el + hl;
+        const int e = (el << 4) + g;
+        const int L = bitrev_dev(e & (half - 1), r);
+        const float *hq = h + q * half;
+        const float xo = tile[ol * kTT + t];
+        const float xa = (t >= L) ? tile[el * kTT + t - L] : hq[half + t - L];
+        const float xb = (t >= L + 1) ? tile[el * kTT + t - L - 1] : hq[half + t - L - 1];
+        o0[i] = kRsqrt2 * (xa + xo);
+        o1[i] = kRsqrt2 * (xb + xo);
+        if (half <= kTT) {
+            sv[i][0] = (t >= kTT - half) ? hist_refill(tile, el, hq, half, nv, t - (kTT - half)) : 0.0f;
+        } else {
+#pragma unroll
+            for (int m = 0; m < HB; m++)
+                sv[i][m] = (m * kTT + t < half) ? hist_refill(tile, el, hq, half, nv, m * kTT + t) : 0.0f;
+        }
+    }
+    sink.read_phase_extra(r);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int idx = i * kTreeThreads + tid;
+        const int q = idx / kTT, t = idx % kTT;
+        const int el = ((q >> rl) << (rl + 1)) + (q & (hl - 1)), ol = el + hl;
+        const int e = (el << 4) + g;
+        float *hq = h + q * half;
+        tile[el * kTT + t] = o0[i];
+        tile[ol * kTT + t] = o1[i];
+        if (half <= kTT) {
+            if (t >= kTT - half)
+                hq[t - (kTT - half)] = sv[i][0];
+        } else {
+#pragma unroll
+            for (int m = 0; m < HB; m++)
+                if (m * kTT + t < half)
+                    hq[m * kTT + t] = sv[i][m];
+        }
+        const int dmb = e & (half - 1);
+        const int p = ((e >> (r + 1)) << r) + dmb;
+        const int s0 = emit[r * 128 + p];
+        if (s0 >= 0) {
+            sink.emit(s0, t, o0[i]);
+            sink.emit(s0 + 1, t, o1[i]);
+        }
+    }
+    sink.write_phase_extra(r);
+    __syncthreads();
+}
+
+template<typename Tst>
+__global__ void __launch_bounds__(kTreeThreads)
+k_stage2_split_b(Stage2Args a, const float *X)
+{
+    extern __shared__ float smem[];
+    float *tile = smem;
+    float *hist = tile + 16 * kTT;
+    float *shist = hist + kSplitHistB;
+    const int tr = blockIdx.x >> 4, g = blockIdx.x & 15, b = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const long gbeam = long(a.beam0) + b;
+    const long nst = kSplitHistB + a.stag_hist_nelts;
+    float *st = a.state + gbeam * a.state_bstride + long(tr) * split_nst(a.stag_hist_nelts)
+                + 16L * kSplitHistA + long(g) * nst;
+    for (int i = tid; i < nst; i += kTreeThreads)
+        hist[i] = st[i];     // hist and shist are contiguous
+
+    const int pf_ndm = 1 << (8 - a.pf_rank);
+    SplitSink<Tst> sink;
+    sink.sbb = reinterpret_cast<Tst *>(a.sb) + gbeam * a.sb_bstride;
+    sink.stag = a.stag;
+    sink.fl = a.fl;
+    sink.stag_begin = a.stag_begin;
+    sink.M = a.M;
+    sink.rowlen = a.rowlen;
+    sink.tpad = a.tpad;
+    sink.dpf_base = bitrev_dev(tr, a.amb) * pf_ndm;
+    sink.g = g;
+    sink.tile = tile;
+    sink.shist = shist;
+    const float *x = X + ((long(b) << a.amb) + tr) * (256L * a.Tp);
+    const int r_lvl0 = 8 - a.pf_rank;
+    __syncthreads();
+
+    const int nlvl0 = a.nlvl0;
+    const int *lvl0_tab = a.lvl0;
+    auto lvl0 = [&sink, tile, tid, g, nlvl0, lvl0_tab]() {
+        for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
+            const int k = idx / kTT, t = idx % kTT;
+            const int row = lvl0_tab[2 * k];
+            if ((row & 15) == g)
+                sink.emit(lvl0_tab[2 * k + 1], t, tile[(row >> 4) * kTT + t]);
+        }
+    };
+
+    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
+        const int nv = min(kTT, a.Tp - t0);
+        sink.t0 = t0;
+        sink.nv = nv;
+        for (int l = __builtin_amdgcn_readfirstlane(tid >> 6); l < 16; l += kTreeThreads / 64)
+            tile[l * kTT + lane] = (lane < nv) ? x[long((l << 4) + g) * a.Tp + t0 + lane] : 0.0f;
+        __syncthreads();
+#pragma unroll 1
+        for (int r = 4; r < 8; r++) {
+            if (r == r_lvl0)
+                lvl0();
+            sink.stag_rounds(r);
+            split_pass(r, g, tile, hist, a.main_emit, sink, nv);
+        }
+        if (r_lvl0 == 8) {
+            lvl0();
+            __syncthreads();
+        }
+    }
+
+    for (int i = tid; i < nst; i += kTreeThreads)
+        st[i] = hist[i];
+}
+
+template<typename Tst>
+static void launch_stage2_split_t(const Stage2Args &a, float *X, hipStream_t st)
+{
+    const dim3 grid{unsigned(16 << a.amb), unsigned(a.B), 1u}, block{unsigned(kTreeThreads), 1u, 1u};
+    hipLaunchKernelGGL((k_stage2_split_a<Tst>), grid, block, 0, st, a, X);
+    const int shm = split_b_lds_bytes(a.stag_hist_nelts);
+    if (shm > 65536)
+        (void) hipFuncSetAttribute((const void *) k_stage2_split_b<Tst>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
+    hipLaunchKernelGGL((k_stage2_split_b<Tst>), grid, block, size_t(shm), st, a, (const float *) X);
+}
+
+void launch_stage2_split(Dtype dt, const Stage2Args &a, float *X, hipStream_t st)
+{
+    switch (dt) {
+        case Dtype::float32: launch_stage2_split_t<float>(a, X, st); break;
+        case Dtype::float16: launch_stage2_split_t<__half>(a, X, st); break;
+        case Dtype::bfloat16: launch_stage2_split_t<__hip_bfloat16>(a, X, st); break;
+    }
 }
 
 template<typename Tst>

@@ -80,6 +80,11 @@ struct Stage2Args {
 long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts);
 int stage2_lds_bytes(int dd, int stag_hist_nelts);
 void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st);
+// Rank-8 stage 2 with subband searches: two launches through a float32 scratch X of
+// (B, 2^amb, 256, Tp) floats; state per tree instance: split_state_nelts_per_tree(...).
+long split_state_nelts_per_tree(int stag_hist_nelts);
+int split_b_lds_bytes(int stag_hist_nelts);
+void launch_stage2_split(Dtype dt, const Stage2Args &a, float *X, hipStream_t st);
 
 // ---- register-resident trees (phase A / LDS delay lines / phase B), bit-identical results ----
 // Stage 1 always; stage 2 when every stream is a final-pass output (no staggered pairs, no

@@ -58,6 +58,18 @@ CASES = {
     'rank16': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=16,
                    time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                    frequency_subband_counts=[1], primary_trees=[pt(0, 8, 64, 64), pt(0, 4, 64, 64)]),
+    # Rank-8 stage 2 with a subband search (chime_sb2-like counts [0,5,7,3,1]) and two early
+    # triggers (ranks 7 and 6 on the LDS-tile kernel): the two-launch split kernel. (The
+    # 'rank15' case covers every ring lag; this one covers the split kernel's passes,
+    # histories and emissions over a few chunks.)
+    'rank15_sb': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=15,
+                      time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                      frequency_subband_counts=[0, 5, 7, 3, 1], primary_trees=[pt(2, 16, 64, 64)]),
+    # Rank-8 stages 1 and 2 with chord_sb2-like counts [5,9,7,3,1] (level-0 copies and every
+    # staggered level), two primary trees, float32.
+    'rank16_sb': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=16,
+                      time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                      frequency_subband_counts=[5, 9, 7, 3, 1], primary_trees=[pt(0, 8, 64, 64), pt(0, 4, 64, 64)]),
     # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
@@ -65,7 +77,7 @@ CASES = {
 }
 
 # Cases whose dense CPU checks (e.g. the analytic-variance oracle) would take minutes.
-LARGE = {'rank15', 'rank16'}
+LARGE = {'rank15', 'rank16', 'rank15_sb', 'rank16_sb'}
 
 # chord16k (bench workload): 16 beams x 16384 channels 300-1500 MHz, rank 14, four
 # primary trees (1/2/4/8 ms), chunks of 2048 samples, float32. Not fixed by the workload and

@@ -226,12 +226,17 @@ def test_steady_state_it0_closed_form():
             assert it0[idm] == -(-(dmax + 4 * t.max_width) // t.time_downsampling)
 
 
-def test_gpu_limits_reported_cleanly():
-    # Stage-2 rank 8 (toplevel rank 16) exceeds this build's GPU kernels: creating a
-    # dedisperser must fail with a clear message, not at launch time.
-    cfg = pa.DedispersionConfig.from_dict(golden()['configs']['chord_sb2_et'])
-    plan = pa.DedispersionPlan(cfg)
-    with pytest.raises(_lib.PddError, match='not supported by this build'):
+def test_reference_configs_accepted_and_limits_reported_cleanly():
+    # Every config of the reference's configs/dedispersion/ (ranks up to 16, subband
+    # searches, early triggers) builds a GPU dedisperser (no device memory is touched before
+    # allocate()); a shape beyond this build's kernels (peak-finder time downsampling 64)
+    # fails at creation with a clear message, not at launch time.
+    for name, d in golden()['configs'].items():
+        pa.GpuDedisperser(pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d)))
+    d = dict(golden()['configs']['chord_sb2_et'])
+    d['primary_trees'] = [dict(t, time_downsampling=64, wt_time_downsampling=64) for t in d['primary_trees']]
+    plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
+    with pytest.raises(_lib.PddError, match='not supported'):
         pa.GpuDedisperser(plan)
 
 
