@@ -37,6 +37,11 @@ namespace {
 
 constexpr int cmin(int a, int b) { return (a < b) ? a : b; }
 
+// Phase-A rows (of 16) of the next tile that rank-7 stage 2 loads during the current tile's
+// phase B. Measured on chord16k (stage2_t0, fp32 / fp16 ms): 0 rows 1.20 / 1.61, 4 rows
+// 1.07 / 1.46, 8 rows 1.07 / 1.52, 12 rows 1.14 / 1.60 (spills in the tile loop), 16 rows 1.31.
+constexpr int kStage2Rank7Prefetch = 4;
+
 __host__ __device__ constexpr int brev_c(int x, int n)
 {
     int r = 0;
@@ -387,14 +392,20 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     // phase-A input rows of tile t0 (issued one tile ahead); PART: a half tile, whose lanes
     // past the chunk end read zeros
     float xa[F::SA][F::NRA];
-    auto load_a = [&](int t0, auto part) {
+    // Stage 1 (and stage 2 below rank 7) issues all of the next tile's phase-A loads right
+    // after this tile's phase A, so they are in flight during phase B; rank-7 stage 2 would
+    // spill with all of them (1.31 vs 1.19 ms on chord16k) and prefetches the first NPF rows.
+    constexpr int NPF = (!STAGE2 || DD <= 6) ? F::NRA : kStage2Rank7Prefetch;
+    // rows [L0, L1) of tile t0
+    auto load_a = [&](int t0, auto part, auto l0, auto l1) {
         constexpr bool PART = decltype(part)::value;
+        constexpr int L0 = decltype(l0)::value, L1 = decltype(l1)::value;
         const bool live = !PART || (t0 + lane < a.Tp);
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             const int h = wid + s * F::NW;
 #pragma unroll
-            for (int l = 0; l < F::NRA; l++) {
+            for (int l = L0; l < L1; l++) {
                 if constexpr (!STAGE2) {
                     xa[s][l] = live ? ld_st(src + long(h * F::NRA + l) * a.Tp + t0 + lane) : 0.0f;
                 } else {
@@ -407,17 +418,18 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     };
     using Full = std::false_type;
     using Half = std::true_type;
+    using R0 = std::integral_constant<int, 0>;
+    using RP = std::integral_constant<int, NPF>;
+    using RN = std::integral_constant<int, F::NRA>;
     const int nfull = a.Tp >> 6;
     const bool has_half = HC && (a.Tp & 63) != 0;
-    // Stage 1 (and stage 2 below rank 7) issues the next tile's phase-A loads right after
-    // this tile's phase A, so they are in flight during phase B; for rank-7 stage 2 the extra
-    // registers would spill, and measured slower (1.31 vs 1.19 ms on chord16k).
-    constexpr bool PREFETCH = !STAGE2 || DD <= 6;
 
     auto tile_step = [&](int t0, auto half) {
         constexpr bool HALF = decltype(half)::value;
-        if constexpr (!PREFETCH || HALF)      // a half tile is never prefetched
-            load_a(t0, half);
+        if constexpr (HALF)                   // a half tile is never prefetched
+            load_a(t0, half, R0{}, RN{});
+        else if constexpr (NPF < F::NRA)
+            load_a(t0, half, RP{}, RN{});
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
             sub_passes<F::K1>(xa[s], ha[s], lane);
@@ -426,9 +438,9 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
                 __builtin_amdgcn_sched_barrier(0);    // one sub-tree at a time (register pressure)
         }
         __syncthreads();
-        if constexpr (PREFETCH && !HALF) {
+        if constexpr (NPF > 0 && !HALF) {
             if (t0 + 128 <= a.Tp)
-                load_a(t0 + 64, Full{});
+                load_a(t0 + 64, Full{}, R0{}, RP{});
         }
 
         const bool live = !HALF || (t0 + lane < a.Tp);
@@ -460,9 +472,9 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
         __syncthreads();
     };
 
-    if constexpr (PREFETCH) {
+    if constexpr (NPF > 0) {
         if (nfull > 0)
-            load_a(0, Full{});
+            load_a(0, Full{}, R0{}, RP{});
     }
     for (int it = 0; it < nfull; it++)
         tile_step(it * 64, Full{});
