@@ -41,3 +41,14 @@ def test_16bit_storage_bit_exact_vs_rounding_oracle(name, dtype):
     from gpu_harness import run_and_compare
     bad, dd, plan = run_and_compare(dict(CASES[name], dtype=dtype), 3, seed=23, log=print)
     assert not bad, f'{len(bad)} mismatching (seq, tree) outputs, first: {bad[0]}'
+
+
+@pytest.mark.parametrize('name,dtype', [('rank16', 'float16'), ('rank16_sb', 'bfloat16'), ('rank15_sb', 'float16')])
+def test_16bit_storage_bit_exact_rank8(name, dtype):
+    """The rank-8 kernels (register tree, and the two-launch split for subband searches with
+    its float32 scratch) with 16-bit storage, against the rounding oracle: 2 chunks."""
+    from gpu_cases import CASES
+    from gpu_harness import run_and_compare
+    print(f'[gpu precision] case={name} dtype={dtype}', flush=True)
+    bad, dd, plan = run_and_compare(dict(CASES[name], dtype=dtype), 2, seed=29, log=print)
+    assert not bad, f'{len(bad)} mismatching (seq, tree) outputs, first: {bad[0]}'
