@@ -266,16 +266,17 @@ k_stage2(Stage2Args a)
 // At rank 8 the LDS tile of k_stage2 would need 256 x 64 floats plus 128 x 255 history floats
 // (191 KB). The tree is split by rows instead: passes 0..3 pair rows inside groups of 16
 // consecutive rows (h = row >> 4), passes 4..7 pair rows with equal low 4 bits (j = row & 15),
-// and every subband stream comes from passes >= dd - pf_rank >= 4. Part A runs passes 0..3 per
-// (tree instance, h) on rows read from the lag ring and writes the tree after pass 3 to a
-// float32 scratch X (B, 2^amb, 256, Tp) of the current chunk; part B runs passes 4..7 per
+// and every subband stream comes from passes >= dd - pf_rank >= 4. Part A (register-tree
+// code, k_tree_fast.hip) runs passes 0..3 per (tree instance, h) on rows read from the lag
+// ring and writes the tree after pass 3 to a float32 scratch X (B, 2^amb, 256, Tp) of the
+// current chunk; part B runs passes 4..7 per
 // (tree instance, j) on X with the full tree's lags, pair histories, staggered butterflies and
 // subband emissions. Every operation is the direct tree's, in its order: bit-identical.
 // A history longer than a tile (2^7 = 128 samples at pass 7) is refilled from the tile and
 // from its own older part.
 
 static constexpr int kSplitNPL = 8;                       // pairs per pass in a 16-row group
-static constexpr int kSplitHistA = kSplitNPL * 15;        // part A pair history (passes 0..3)
+static constexpr int kSplitHistA = kSplitNPL * 15;        // part A pair history (passes 0..3, k_tree_fast.hip)
 static constexpr int kSplitHistB = kSplitNPL * (256 - 16);   // part B pair history (passes 4..7)
 
 __host__ __device__ static inline long split_nst(int stag_hist_nelts)
@@ -285,61 +286,11 @@ __host__ __device__ static inline long split_nst(int stag_hist_nelts)
 
 long split_state_nelts_per_tree(int stag_hist_nelts) { return split_nst(stag_hist_nelts); }
 
-struct NoSink {
-    __device__ void read_phase_extra(int) { }
-    __device__ void write_phase_extra(int) { }
-    __device__ void emit(int, int, float) { }
-};
+
 
 int split_b_lds_bytes(int stag_hist_nelts)
 {
     return int(sizeof(float) * (16 * kTT + kSplitHistB + stag_hist_nelts));
-}
-
-template<typename Tst>
-__global__ void __launch_bounds__(kTreeThreads)
-k_stage2_split_a(Stage2Args a, float *X)
-{
-    using TD = TreeDims<4>;
-    __shared__ float tile[16 * kTT];
-    __shared__ float hist[TD::HIST];
-    const int tr = blockIdx.x >> 4, h = blockIdx.x & 15, b = blockIdx.y;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const long gbeam = long(a.beam0) + b;
-    float *st = a.state + gbeam * a.state_bstride + long(tr) * split_nst(a.stag_hist_nelts)
-                + long(h) * kSplitHistA;
-    for (int i = tid; i < TD::HIST; i += kTreeThreads)
-        hist[i] = st[i];
-    const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + gbeam * a.ring.bstride;
-    float *x = X + ((long(b) << a.amb) + tr) * (256L * a.Tp);
-    NoSink sink;
-    __syncthreads();
-
-    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
-        const int nv = min(kTT, a.Tp - t0);
-        const long tau0 = a.chunk * a.Tp + t0;
-        for (int l = __builtin_amdgcn_readfirstlane(tid >> 6); l < 16; l += kTreeThreads / 64) {
-            const long rr = (long(h * 16 + l) << a.amb) + tr;
-            const long cap = a.ring.row_cap[rr];
-            long pos = ring_pos(tau0 - a.row_lag[rr], cap) + lane;
-            if (pos >= cap)
-                pos -= cap;
-            tile[l * kTT + lane] = ld_st(ring + a.ring.row_off[rr] + pos);
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int r = 0; r < 4; r++)
-            tree_pass<4>(r, tile, hist, nullptr, sink, nv);
-        for (int idx = tid; idx < 16 * kTT; idx += kTreeThreads) {
-            const int l = idx / kTT, t = idx % kTT;
-            if (t < nv)
-                x[long(h * 16 + l) * a.Tp + t0 + t] = tile[idx];
-        }
-        __syncthreads();
-    }
-
-    for (int i = tid; i < TD::HIST; i += kTreeThreads)
-        st[i] = hist[i];
 }
 
 // New content of history slot k (time nv - half + k relative to this tile) of a row whose
@@ -570,7 +521,6 @@ template<typename Tst>
 static void launch_stage2_split_t(const Stage2Args &a, float *X, hipStream_t st)
 {
     const dim3 grid{unsigned(16 << a.amb), unsigned(a.B), 1u}, block{unsigned(kTreeThreads), 1u, 1u};
-    hipLaunchKernelGGL((k_stage2_split_a<Tst>), grid, block, 0, st, a, X);
     const int shm = split_b_lds_bytes(a.stag_hist_nelts);
     if (shm > 65536)
         (void) hipFuncSetAttribute((const void *) k_stage2_split_b<Tst>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);
@@ -579,6 +529,7 @@ static void launch_stage2_split_t(const Stage2Args &a, float *X, hipStream_t st)
 
 void launch_stage2_split(Dtype dt, const Stage2Args &a, float *X, hipStream_t st)
 {
+    launch_stage2_split_a_reg(dt, a, X, split_nst(a.stag_hist_nelts), st);
     switch (dt) {
         case Dtype::float32: launch_stage2_split_t<float>(a, X, st); break;
         case Dtype::float16: launch_stage2_split_t<__half>(a, X, st); break;

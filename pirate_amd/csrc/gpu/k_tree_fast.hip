@@ -573,6 +573,75 @@ void launch_stage1_fast(Dtype dt, const Stage1Args &a, hipStream_t st)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Part A of the rank-8 split stage 2 (k_tree.hip): passes 0..3 of the rank-8 tree, which
+// pair rows inside each group of 16 consecutive rows, are a rank-4 tree per group. One wave
+// per (tree instance tr, group h) keeps the 16 rows in registers (the register kernel's
+// sub-tree passes: the direct tree's operations, bit for bit), reads them from the lag ring
+// at their stage-2 lags and writes the tree after pass 3 to the float32 scratch X
+// (B, 2^amb, 256, Tp). Pass histories are saved in the same layout as the LDS-tile
+// kernels' (8 pairs x (1 + 2 + 4 + 8) floats per group), after the rank-8 split's state.
+
+static constexpr int kSplitHistA = 8 * 15;
+
+template<typename Tst>
+__global__ void __launch_bounds__(256)
+k_stage2_split_a_reg(Stage2Args a, float *X, long split_nst)
+{
+    using SA_ = Sub<4>;
+    const int lane = threadIdx.x & 63;
+    const int grp = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tr = grp >> 4, h = grp & 15, b = blockIdx.y;
+    const long gbeam = long(a.beam0) + b;
+    float *st = a.state + gbeam * a.state_bstride + long(tr) * split_nst + long(h) * kSplitHistA;
+
+    float hh[SA_::NH];
+    load_hist<4>(hh, st, lane);
+
+    // per-row ring position of this chunk's time 0 (wave-uniform)
+    const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + gbeam * a.ring.bstride;
+    long rbase[16];
+    int rwrap[16], rcap[16];
+#pragma unroll
+    for (int l = 0; l < 16; l++) {
+        const long rr = (long(h * 16 + l) << a.amb) + tr;
+        const long cap = a.ring.row_cap[rr];
+        const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+        rbase[l] = a.ring.row_off[rr] + pos;
+        rwrap[l] = int(cap - pos);
+        rcap[l] = int(cap);
+    }
+    float *x = X + ((long(b) << a.amb) + tr) * (256L * a.Tp) + long(h) * 16 * a.Tp;
+
+    for (int t0 = 0; t0 < a.Tp; t0 += 64) {
+        const int tl = t0 + lane;
+        const bool live = tl < a.Tp;
+        float v[16];
+#pragma unroll
+        for (int l = 0; l < 16; l++)
+            v[l] = live ? ld_st(ring + (rbase[l] + tl - ((tl >= rwrap[l]) ? rcap[l] : 0))) : 0.0f;
+        sub_passes<4>(v, hh, lane);
+        if (live) {
+#pragma unroll
+            for (int l = 0; l < 16; l++)
+                x[long(l) * a.Tp + tl] = v[l];
+        }
+    }
+    save_hist<4>(hh, st, lane, (a.Tp & 63) ? (a.Tp & 63) : 64);
+}
+
+void launch_stage2_split_a_reg(Dtype dt, const Stage2Args &a, float *X, long split_nst, hipStream_t st)
+{
+    const dim3 grid{unsigned((16 << a.amb) / 4), unsigned(a.B), 1u}, block{256u, 1u, 1u};
+    switch (dt) {
+        case Dtype::float32: hipLaunchKernelGGL(k_stage2_split_a_reg<float>, grid, block, 0, st, a, X, split_nst); break;
+        case Dtype::float16: hipLaunchKernelGGL(k_stage2_split_a_reg<__half>, grid, block, 0, st, a, X, split_nst); break;
+        case Dtype::bfloat16:
+            hipLaunchKernelGGL(k_stage2_split_a_reg<__hip_bfloat16>, grid, block, 0, st, a, X, split_nst);
+            break;
+    }
+}
+
 void launch_stage2_fast(Dtype dt, const Stage2Args &a, hipStream_t st)
 {
     switch (dt) {

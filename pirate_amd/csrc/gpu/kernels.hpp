@@ -85,6 +85,8 @@ void launch_stage2(Dtype dt, const Stage2Args &a, hipStream_t st);
 long split_state_nelts_per_tree(int stag_hist_nelts);
 int split_b_lds_bytes(int stag_hist_nelts);
 void launch_stage2_split(Dtype dt, const Stage2Args &a, float *X, hipStream_t st);
+// its part A (passes 0..3) on the register-tree code, one wave per 16-row group
+void launch_stage2_split_a_reg(Dtype dt, const Stage2Args &a, float *X, long split_nst, hipStream_t st);
 
 // ---- register-resident trees (phase A / LDS delay lines / phase B), bit-identical results ----
 // Stage 1 always; stage 2 when every stream is a final-pass output (no staggered pairs, no
