@@ -197,7 +197,12 @@ void GpuDedisperser::build_tables()
 
         T.tpad = int(std::max(2 * t.pf.max_width, 4L));
         T.nlev = std::max(integer_log2(t.pf.max_width), 1);
-        T.rowlen = int(align_up(T.tpad + t.nt_ds + T.flmax, 4));   // 16-byte aligned rows (vector loads)
+        // Rows start on 128-byte lines and the buffer is offset by 'lead' elements so that
+        // chunk sample 0 (position tpad) of every row starts a line: the stage-2 tile stores
+        // and the peak finder's loads then cover whole lines.
+        const int line_elts = int(128 / elt_bytes);
+        T.lead = int(align_up(T.tpad, line_elts)) - T.tpad;
+        T.rowlen = int(align_up(T.tpad + t.nt_ds + T.flmax, line_elts));
     }
 }
 
@@ -432,7 +437,7 @@ void GpuDedisperser::allocate()
         // across chunks: one set of rows per beam of the GPU, not per batch.
         PDD_CHECK(t.ndm_out * T.M * T.rowlen < (1L << 31), "GpuDedisperser: peak-finder rows of tree " << it
                   << " exceed 2^31 elements per beam");
-        sb[it] = dalloc(size_t(total_beams) * t.ndm_out * T.M * T.rowlen * elt_bytes);
+        sb[it] = dalloc((size_t(total_beams) * t.ndm_out * T.M * T.rowlen + T.lead) * elt_bytes);
         d_main_emit[it] = up_int(T.main_emit);
         d_stag[it] = up_int(T.stag);
         d_lvl0[it] = up_int(T.lvl0);
@@ -581,7 +586,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
         s.ring.bstride = ring_bstride[p];
         s.row_lag = (const int *) stage2_lags[it].ptr;
-        s.sb = sb[it].ptr;
+        s.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
         s.sb_bstride = t.ndm_out * T.M * T.rowlen;
         s.rowlen = T.rowlen;
         s.tpad = T.tpad;
@@ -617,7 +622,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         timed_end(nm2, st, ev);
 
         PfArgs f;
-        f.sb = sb[it].ptr;
+        f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
         f.sb_bstride = s.sb_bstride;
         f.rowlen = T.rowlen;
         f.tpad = T.tpad;

@@ -111,6 +111,7 @@ public:
         bool fast = false;             // stage 2 runs on the register-tree kernel
         bool split = false;            // rank-8 subband search: two-launch split kernel
         int nstag = 0, nlvl0 = 0, stag_hist_nelts = 0, flmax = 0, tpad = 0, nlev = 0, rowlen = 0, M = 0, N = 0;
+        int lead = 0;                  // element offset of the peak-finder rows in their buffer
     };
     std::vector<TreeTables> tables;
 
