@@ -67,7 +67,7 @@ GpuDedisperser::~GpuDedisperser()
             (void) hipHostFree(h);
     for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &ring1_row_off, &ring1_row_cap, &spill_rows,
                     &stage2_lags, &sb, &d_main_emit, &d_stag,
-                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_out_stream, &s1_state, &s2_state, &weights, &out_var})
+                    &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_stag_goff, &d_out_stream, &s1_state, &s2_state, &weights, &out_var})
         for (auto &a : *v)
             freeit(a);
     freeit(out_region);     // out_max / out_argmax are views into it
@@ -158,6 +158,18 @@ void GpuDedisperser::build_tables()
         }
         T.nlvl0 = int(T.lvl0.size() / 2);
         T.stag_hist_nelts = hoff;
+        if (dd == 8) {
+            // Rank-8 split: each group of rows with equal low 4 bits keeps only the histories
+            // of its own staggered pairs (group = row0 & 15), packed.
+            std::vector<int> gtot(16, 0);
+            T.stag_goff.assign(T.nstag, 0);
+            for (int k = 0; k < T.nstag; k++) {
+                const int g = T.stag[6 * k + 1] & 15;
+                T.stag_goff[k] = gtot[g];
+                gtot[g] += 1 << T.stag[6 * k];
+            }
+            T.stag_hist_g = *std::max_element(gtot.begin(), gtot.end());
+        }
 
         // Register-tree path: every stream is a final-pass output (main pairs of the last
         // pass, or level-0 copies after the last pass), each row feeding at most one stream.
@@ -234,7 +246,7 @@ void GpuDedisperser::validate_gpu_limits()
                       "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
         if (!T.fast && t.dd_rank == 8) {
             PDD_CHECK(t.frequency_subbands.pf_rank <= 4, "GpuDedisperser: tree " << it << ": rank-8 split needs pf_rank <= 4");
-            PDD_CHECK(split_b_lds_bytes(T.stag_hist_nelts) <= kLdsBytesPerCU,
+            PDD_CHECK(split_b_lds_bytes(T.stag_hist_g) <= kLdsBytesPerCU,
                       "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
         }
         PDD_CHECK(t.pf.time_downsampling <= 512, "GpuDedisperser: time_downsampling too large");
@@ -415,6 +427,7 @@ void GpuDedisperser::allocate()
     d_fl.resize(ntrees);
     d_m_to_n.resize(ntrees);
     d_stag_begin.resize(ntrees);
+    d_stag_goff.resize(ntrees);
     d_out_stream.resize(ntrees);
     s2_state.resize(ntrees);
     s2_state_bstride.resize(ntrees);
@@ -444,9 +457,10 @@ void GpuDedisperser::allocate()
         d_fl[it] = up_int(T.fl);
         d_m_to_n[it] = up_int(T.m_to_n);
         d_stag_begin[it] = up_int(T.stag_begin);
+        d_stag_goff[it] = up_int(T.stag_goff);
         d_out_stream[it] = up_int(T.out_stream);
         s2_state_bstride[it] = pow2(t.amb_rank) * (T.fast ? fast_tree_state_nelts(t.dd_rank)
-                                                   : T.split ? split_state_nelts_per_tree(T.stag_hist_nelts)
+                                                   : T.split ? split_state_nelts_per_tree(T.stag_hist_g)
                                                              : stage2_state_nelts_per_tree(t.dd_rank, T.stag_hist_nelts));
         if (T.split)
             split_nelts = std::max(split_nelts, size_t(B) * pow2(t.amb_rank) * 256 * t.nt_ds);
@@ -596,7 +610,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.fl = (const int *) d_fl[it].ptr;
         s.nstag = T.nstag;
         s.nlvl0 = T.nlvl0;
-        s.stag_hist_nelts = T.stag_hist_nelts;
+        s.stag_hist_nelts = T.split ? T.stag_hist_g : T.stag_hist_nelts;
+        s.stag_goff = (const int *) d_stag_goff[it].ptr;
         s.stag_begin = (const int *) d_stag_begin[it].ptr;
         s.M = T.M;
         s.pf_rank = int(t.frequency_subbands.pf_rank);

@@ -112,6 +112,8 @@ public:
         bool split = false;            // rank-8 subband search: two-launch split kernel
         int nstag = 0, nlvl0 = 0, stag_hist_nelts = 0, flmax = 0, tpad = 0, nlev = 0, rowlen = 0, M = 0, N = 0;
         int lead = 0;                  // element offset of the peak-finder rows in their buffer
+        std::vector<int> stag_goff;    // rank 8: staggered history offset within its row group
+        int stag_hist_g = 0;           // rank 8: staggered history floats per row group
     };
     std::vector<TreeTables> tables;
 
@@ -150,7 +152,7 @@ private:
     std::vector<void *> host_ring;                           // [p] pinned, (total_beams, host_bstride)
     std::vector<DeviceArray> stage2_lags;    // [itree] device int
     std::vector<DeviceArray> sb;      // [itree] (B, ndm_out, M, rowlen)
-    std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin, d_out_stream;
+    std::vector<DeviceArray> d_main_emit, d_stag, d_lvl0, d_fl, d_m_to_n, d_stag_begin, d_stag_goff, d_out_stream;
     DeviceArray lds_state;  long lds_state_bstride = 0;
     std::vector<DeviceArray> s1_state; std::vector<long> s1_state_bstride;
     std::vector<DeviceArray> s2_state; std::vector<long> s2_state_bstride;

@@ -278,6 +278,7 @@ k_stage2(Stage2Args a)
 static constexpr int kSplitNPL = 8;                       // pairs per pass in a 16-row group
 static constexpr int kSplitHistA = kSplitNPL * 15;        // part A pair history (passes 0..3, k_tree_fast.hip)
 static constexpr int kSplitHistB = kSplitNPL * (256 - 16);   // part B pair history (passes 4..7)
+static constexpr int kSplitGPB = 4;                       // part B row groups per workgroup
 
 __host__ __device__ static inline long split_nst(int stag_hist_nelts)
 {
@@ -288,9 +289,9 @@ long split_state_nelts_per_tree(int stag_hist_nelts) { return split_nst(stag_his
 
 
 
-int split_b_lds_bytes(int stag_hist_nelts)
+int split_b_lds_bytes(int stag_hist_g)
 {
-    return int(sizeof(float) * (16 * kTT + kSplitHistB + stag_hist_nelts));
+    return int(sizeof(float) * (kSplitGPB * (16 * kTT + kSplitHistB + stag_hist_g)));
 }
 
 // New content of history slot k (time nv - half + k relative to this tile) of a row whose
@@ -301,14 +302,19 @@ __device__ __forceinline__ float hist_refill(const float *tile, int row, const f
     return (tm >= 0) ? tile[row * kTT + tm] : old[nv + k];
 }
 
+// Part B handles kSplitGPB row groups (equal low 4 bits g0..g0+GPB-1) per workgroup: tile
+// rows [GPB][16][64]; per group in LDS, in the state buffer's layout: pair histories
+// (kSplitHistB floats) then the group's own staggered histories (stag_hist_g floats, offsets
+// stag_goff[k]).
+
 // Part-B emissions: subband streams into the peak-finder rows, with staggered butterflies
-// and level-0 copies restricted to the group's rows (global row & 15 == g).
+// and level-0 copies restricted to the workgroup's groups.
 template<typename Tst>
 struct SplitSink {
     Tst *sbb;
-    const int *stag, *fl, *stag_begin;
-    int M, rowlen, tpad, dpf_base, t0, nv, g;
-    float *tile, *shist;
+    const int *stag, *fl, *stag_begin, *stag_goff;
+    int M, rowlen, tpad, dpf_base, t0, nv, g0, nstg;
+    float *tile, *stl;
     static constexpr int SMAX = 4, HB = 2;     // entries per thread per round; 64-slot history blocks
 
     __device__ __forceinline__ void emit(int s, int t, float v)
@@ -320,11 +326,8 @@ struct SplitSink {
         st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
     }
 
-    __device__ void read_phase_extra(int) { }
-    __device__ void write_phase_extra(int) { }
-
-    // Staggered butterflies of pass r whose rows are in group g, before the pass (as in
-    // Stage2Sink::stag_rounds).
+    // Staggered butterflies of pass r whose rows are in this workgroup's groups, before the
+    // pass (as in Stage2Sink::stag_rounds).
     __device__ void stag_rounds(int r)
     {
         const int half = 1 << r;
@@ -337,22 +340,24 @@ struct SplitSink {
                 if (idx < n) {
                     const int k = k0 + idx / kTT, t = idx % kTT;
                     const int *e = stag + 6 * k;
-                    if ((e[1] & 15) == g) {
+                    const unsigned gi = unsigned((e[1] & 15) - g0);
+                    if (gi < unsigned(kSplitGPB)) {
+                        const float *tg = tile + gi * (16 * kTT);
                         const int row0 = e[1] >> 4, row1 = e[2] >> 4, L = e[3];
-                        const float *h = shist + e[5];
-                        const float xo = tile[row1 * kTT + t];
-                        const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
-                        const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
+                        const float *h = stl + gi * nstg + kSplitHistB + stag_goff[k];
+                        const float xo = tg[row1 * kTT + t];
+                        const float xa = (t >= L) ? tg[row0 * kTT + t - L] : h[half + t - L];
+                        const float xb = (t >= L + 1) ? tg[row0 * kTT + t - L - 1] : h[half + t - L - 1];
                         so0[q] = kRsqrt2 * (xa + xo);
                         so1[q] = kRsqrt2 * (xb + xo);
                         if (half <= kTT) {
                             if (t >= kTT - half)
-                                ssv[q][0] = hist_refill(tile, row0, h, half, nv, t - (kTT - half));
+                                ssv[q][0] = hist_refill(tg, row0, h, half, nv, t - (kTT - half));
                         } else {
 #pragma unroll
                             for (int m = 0; m < HB; m++)
                                 if (m * kTT + t < half)
-                                    ssv[q][m] = hist_refill(tile, row0, h, half, nv, m * kTT + t);
+                                    ssv[q][m] = hist_refill(tg, row0, h, half, nv, m * kTT + t);
                         }
                     }
                 }
@@ -364,10 +369,11 @@ struct SplitSink {
                 if (idx < n) {
                     const int k = k0 + idx / kTT, t = idx % kTT;
                     const int *e = stag + 6 * k;
-                    if ((e[1] & 15) == g) {
+                    const unsigned gi = unsigned((e[1] & 15) - g0);
+                    if (gi < unsigned(kSplitGPB)) {
                         emit(e[4], t, so0[q]);
                         emit(e[4] + 1, t, so1[q]);
-                        float *h = shist + e[5];
+                        float *h = stl + gi * nstg + kSplitHistB + stag_goff[k];
                         if (half <= kTT) {
                             if (t >= kTT - half)
                                 h[t - (kTT - half)] = ssv[q][0];
@@ -385,48 +391,52 @@ struct SplitSink {
     }
 };
 
-// Pass r (4..7) of the rank-8 tree on the 16 rows of group g (tile row l = global row >> 4).
+// Pass r (4..7) of the rank-8 tree on the workgroup's groups g0 + gi (tile row l of group gi
+// = global row (l << 4) + g0 + gi).
 template<typename Sink>
-__device__ __forceinline__ void split_pass(int r, int g, float *tile, float *hist, const int *emit, Sink &sink, int nv)
+__device__ __forceinline__ void split_pass(int r, int g0, float *tile, float *stl, int nstg, const int *emit, Sink &sink,
+                                           int nv)
 {
-    constexpr int NPL = kSplitNPL, ITEMS = NPL * kTT / kTreeThreads, HB = 2;
+    constexpr int NPL = kSplitNPL, PER_G = NPL * kTT, ITEMS = kSplitGPB * PER_G / kTreeThreads, HB = 2;
     const int tid = threadIdx.x;
     const int rl = r - 4, hl = 1 << rl, half = 1 << r;
-    float *h = hist + NPL * (half - 16);
 
     float o0[ITEMS], o1[ITEMS], sv[ITEMS][HB];
 #pragma unroll
     for (int i = 0; i < ITEMS; i++) {
         const int idx = i * kTreeThreads + tid;
-        const int q = idx / kTT, t = idx % kTT;
+        const int gi = idx / PER_G, rem = idx - gi * PER_G;
+        const int q = rem / kTT, t = rem % kTT;
+        const float *tg = tile + gi * (16 * kTT);
         const int el = ((q >> rl) << (rl + 1)) + (q & (hl - 1)), ol = el + hl;
-        const int e = (el << 4) + g;
+        const int e = (el << 4) + g0 + gi;
         const int L = bitrev_dev(e & (half - 1), r);
-        const float *hq = h + q * half;
-        const float xo = tile[ol * kTT + t];
-        const float xa = (t >= L) ? tile[el * kTT + t - L] : hq[half + t - L];
-        const float xb = (t >= L + 1) ? tile[el * kTT + t - L - 1] : hq[half + t - L - 1];
+        const float *hq = stl + gi * nstg + NPL * (half - 16) + q * half;
+        const float xo = tg[ol * kTT + t];
+        const float xa = (t >= L) ? tg[el * kTT + t - L] : hq[half + t - L];
+        const float xb = (t >= L + 1) ? tg[el * kTT + t - L - 1] : hq[half + t - L - 1];
         o0[i] = kRsqrt2 * (xa + xo);
         o1[i] = kRsqrt2 * (xb + xo);
         if (half <= kTT) {
-            sv[i][0] = (t >= kTT - half) ? hist_refill(tile, el, hq, half, nv, t - (kTT - half)) : 0.0f;
+            sv[i][0] = (t >= kTT - half) ? hist_refill(tg, el, hq, half, nv, t - (kTT - half)) : 0.0f;
         } else {
 #pragma unroll
             for (int m = 0; m < HB; m++)
-                sv[i][m] = (m * kTT + t < half) ? hist_refill(tile, el, hq, half, nv, m * kTT + t) : 0.0f;
+                sv[i][m] = (m * kTT + t < half) ? hist_refill(tg, el, hq, half, nv, m * kTT + t) : 0.0f;
         }
     }
-    sink.read_phase_extra(r);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; i++) {
         const int idx = i * kTreeThreads + tid;
-        const int q = idx / kTT, t = idx % kTT;
+        const int gi = idx / PER_G, rem = idx - gi * PER_G;
+        const int q = rem / kTT, t = rem % kTT;
+        float *tg = tile + gi * (16 * kTT);
         const int el = ((q >> rl) << (rl + 1)) + (q & (hl - 1)), ol = el + hl;
-        const int e = (el << 4) + g;
-        float *hq = h + q * half;
-        tile[el * kTT + t] = o0[i];
-        tile[ol * kTT + t] = o1[i];
+        const int e = (el << 4) + g0 + gi;
+        float *hq = stl + gi * nstg + NPL * (half - 16) + q * half;
+        tg[el * kTT + t] = o0[i];
+        tg[ol * kTT + t] = o1[i];
         if (half <= kTT) {
             if (t >= kTT - half)
                 hq[t - (kTT - half)] = sv[i][0];
@@ -444,7 +454,6 @@ __device__ __forceinline__ void split_pass(int r, int g, float *tile, float *his
             sink.emit(s0 + 1, t, o1[i]);
         }
     }
-    sink.write_phase_extra(r);
     __syncthreads();
 }
 
@@ -452,18 +461,18 @@ template<typename Tst>
 __global__ void __launch_bounds__(kTreeThreads)
 k_stage2_split_b(Stage2Args a, const float *X)
 {
+    constexpr int GPB = kSplitGPB;
     extern __shared__ float smem[];
-    float *tile = smem;
-    float *hist = tile + 16 * kTT;
-    float *shist = hist + kSplitHistB;
-    const int tr = blockIdx.x >> 4, g = blockIdx.x & 15, b = blockIdx.y;
+    float *tile = smem;                      // [GPB][16][kTT]
+    float *stl = tile + GPB * 16 * kTT;      // [GPB][nstg]
+    const int tr = blockIdx.x / (16 / GPB), g0 = (blockIdx.x % (16 / GPB)) * GPB, b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63;
     const long gbeam = long(a.beam0) + b;
-    const long nst = kSplitHistB + a.stag_hist_nelts;
+    const int nstg = kSplitHistB + a.stag_hist_nelts;     // stag_hist_nelts = per-group size here
     float *st = a.state + gbeam * a.state_bstride + long(tr) * split_nst(a.stag_hist_nelts)
-                + 16L * kSplitHistA + long(g) * nst;
-    for (int i = tid; i < nst; i += kTreeThreads)
-        hist[i] = st[i];     // hist and shist are contiguous
+                + 16L * kSplitHistA + long(g0) * nstg;
+    for (int i = tid; i < GPB * nstg; i += kTreeThreads)
+        stl[i] = st[i];
 
     const int pf_ndm = 1 << (8 - a.pf_rank);
     SplitSink<Tst> sink;
@@ -471,25 +480,28 @@ k_stage2_split_b(Stage2Args a, const float *X)
     sink.stag = a.stag;
     sink.fl = a.fl;
     sink.stag_begin = a.stag_begin;
+    sink.stag_goff = a.stag_goff;
     sink.M = a.M;
     sink.rowlen = a.rowlen;
     sink.tpad = a.tpad;
     sink.dpf_base = bitrev_dev(tr, a.amb) * pf_ndm;
-    sink.g = g;
+    sink.g0 = g0;
+    sink.nstg = nstg;
     sink.tile = tile;
-    sink.shist = shist;
+    sink.stl = stl;
     const float *x = X + ((long(b) << a.amb) + tr) * (256L * a.Tp);
     const int r_lvl0 = 8 - a.pf_rank;
     __syncthreads();
 
     const int nlvl0 = a.nlvl0;
     const int *lvl0_tab = a.lvl0;
-    auto lvl0 = [&sink, tile, tid, g, nlvl0, lvl0_tab]() {
+    auto lvl0 = [&sink, tile, tid, g0, nlvl0, lvl0_tab]() {
         for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
             const int k = idx / kTT, t = idx % kTT;
             const int row = lvl0_tab[2 * k];
-            if ((row & 15) == g)
-                sink.emit(lvl0_tab[2 * k + 1], t, tile[(row >> 4) * kTT + t]);
+            const unsigned gi = unsigned((row & 15) - g0);
+            if (gi < unsigned(GPB))
+                sink.emit(lvl0_tab[2 * k + 1], t, tile[gi * (16 * kTT) + (row >> 4) * kTT + t]);
         }
     };
 
@@ -497,15 +509,17 @@ k_stage2_split_b(Stage2Args a, const float *X)
         const int nv = min(kTT, a.Tp - t0);
         sink.t0 = t0;
         sink.nv = nv;
-        for (int l = __builtin_amdgcn_readfirstlane(tid >> 6); l < 16; l += kTreeThreads / 64)
-            tile[l * kTT + lane] = (lane < nv) ? x[long((l << 4) + g) * a.Tp + t0 + lane] : 0.0f;
+        for (int rw = __builtin_amdgcn_readfirstlane(tid >> 6); rw < GPB * 16; rw += kTreeThreads / 64) {
+            const int gi = rw >> 4, l = rw & 15;
+            tile[rw * kTT + lane] = (lane < nv) ? x[long((l << 4) + g0 + gi) * a.Tp + t0 + lane] : 0.0f;
+        }
         __syncthreads();
 #pragma unroll 1
         for (int r = 4; r < 8; r++) {
             if (r == r_lvl0)
                 lvl0();
             sink.stag_rounds(r);
-            split_pass(r, g, tile, hist, a.main_emit, sink, nv);
+            split_pass(r, g0, tile, stl, nstg, a.main_emit, sink, nv);
         }
         if (r_lvl0 == 8) {
             lvl0();
@@ -513,14 +527,14 @@ k_stage2_split_b(Stage2Args a, const float *X)
         }
     }
 
-    for (int i = tid; i < nst; i += kTreeThreads)
-        st[i] = hist[i];
+    for (int i = tid; i < GPB * nstg; i += kTreeThreads)
+        st[i] = stl[i];
 }
 
 template<typename Tst>
 static void launch_stage2_split_t(const Stage2Args &a, float *X, hipStream_t st)
 {
-    const dim3 grid{unsigned(16 << a.amb), unsigned(a.B), 1u}, block{unsigned(kTreeThreads), 1u, 1u};
+    const dim3 grid{unsigned((16 / kSplitGPB) << a.amb), unsigned(a.B), 1u}, block{unsigned(kTreeThreads), 1u, 1u};
     const int shm = split_b_lds_bytes(a.stag_hist_nelts);
     if (shm > 65536)
         (void) hipFuncSetAttribute((const void *) k_stage2_split_b<Tst>, hipFuncAttributeMaxDynamicSharedMemorySize, shm);

@@ -76,6 +76,7 @@ struct Stage2Args {
     int B, amb, dd, Tp, beam0, amb1_dd1;   // amb1_dd1: stage-1 dd rank (ring row = fc*2^amb1_dd1 + j)
     long chunk;
     const int *out_stream;    // register-tree path: (2^dd) final row -> stream, or -1
+    const int *stag_goff;     // rank-8 split: staggered history offset within its row group
 };
 long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts);
 int stage2_lds_bytes(int dd, int stag_hist_nelts);
