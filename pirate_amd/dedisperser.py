@@ -332,3 +332,16 @@ class OutputRingView:
         if self._base is not None and self._base.value:
             L.check(L.lib().pdd_ipc_close(self._base, self.device))
         self._base = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, '_base', None) is not None and L._lib is not None:
+            try:
+                self.close()
+            except Exception:     # noqa: BLE001 -- interpreter shutdown
+                pass
