@@ -22,8 +22,13 @@
 
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "tree_regs.hpp"
+
+#include <type_traits>
 
 namespace pirate_amd {
+
+using namespace treg;
 
 static constexpr int kTT = 64;
 static constexpr int kTreeThreads = 512;   // 8 waves: 35% faster than 4 on the rank-7 subband kernel
@@ -110,8 +115,19 @@ long stage2_state_nelts_per_tree(int dd, int stag_hist_nelts)
 int stage2_lds_bytes(int dd, int stag_hist_nelts)
 {
     long d = 1L << dd;
-    return int(sizeof(float) * (d * kTT + (d / 2) * (d - 1) + stag_hist_nelts));
+    return int(sizeof(float) * (d * kTT + (d / 2) * (d - 1) + stag_hist_nelts + 3) + 16 * d);   // + row descriptors
 }
+
+// Register pre-passes of k_stage2: passes 0..KP-1 (KP <= 4, all before the first subband
+// emission) pair rows inside groups of 2^KP consecutive rows, so each group is a rank-KP
+// sub-tree that one wave runs in registers (tree_regs.hpp) straight from the ring loads,
+// before the tile goes to LDS for the remaining passes. Groups per wave:
+template<int DD, int KP>
+struct PrePass {
+    static constexpr int NG = (1 << DD) >> KP;                          // groups
+    static constexpr int NWV = kTreeThreads / 64;
+    static constexpr int GPW = (NG + NWV - 1) / NWV;                    // groups per wave
+};
 
 // Emits subband stream samples (and runs the staggered butterflies and level-0 copies)
 // into the peak-finder input rows. Holds copies of the fields it needs (not a reference
@@ -192,14 +208,24 @@ k_stage2(Stage2Args a)
     float *tile = smem;
     float *hist = smem + TD::D * kTT;
     float *shist = hist + TD::HIST;
+    const long nst = TD::HIST + a.stag_hist_nelts;
+    int4 *desc = reinterpret_cast<int4 *>(smem + ((TD::D * kTT + nst + 3) & ~3L));   // 16-byte aligned
 
     const int j = blockIdx.x, b = blockIdx.y;
-    const int tid = threadIdx.x;
-    const long nst = TD::HIST + a.stag_hist_nelts;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     float *st = a.state + long(a.beam0 + b) * a.state_bstride + long(j) * nst;
 
     for (int i = tid; i < nst; i += kTreeThreads)
         hist[i] = st[i];     // hist and shist are contiguous
+    // ring row of tree row fc at this chunk's time 0: (offset + position, samples to the
+    // wrap, capacity)
+    for (int fc = tid; fc < TD::D; fc += kTreeThreads) {
+        const long rr = (long(fc) << a.amb) + j;
+        const long cap = a.ring.row_cap[rr];
+        const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+        desc[fc] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
+    }
 
     const int pf_ndm = 1 << (DD - a.pf_rank);
     const int dpf_base = bitrev_dev(j, a.amb) * pf_ndm;
@@ -219,41 +245,115 @@ k_stage2(Stage2Args a)
     sink.shist = shist;
     sink.stag_begin = a.stag_begin;
     const int r_lvl0 = DD - a.pf_rank;
+    const int nlvl0 = a.nlvl0;
+    const int *lvl0_tab = a.lvl0;
+    const int *main_emit = a.main_emit;
+    const int Tp = a.Tp;
+    __syncthreads();
 
-    for (int t0 = 0; t0 < a.Tp; t0 += kTT) {
-        const int nv = min(kTT, a.Tp - t0);
-        sink.t0 = t0;
-        sink.nv = nv;
-        const long tau0 = a.chunk * a.Tp + t0;
-        const int lane = tid & 63;
-        for (int fc = __builtin_amdgcn_readfirstlane(tid >> 6); fc < TD::D; fc += kTreeThreads / 64) {
-            const long rr = (long(fc) << a.amb) + j;
-            const long cap = a.ring.row_cap[rr];
-            long pos = ring_pos(tau0 - a.row_lag[rr], cap) + lane;
-            if (pos >= cap)
-                pos -= cap;
-            tile[fc * kTT + lane] = ld_st(ring + a.ring.row_off[rr] + pos);
-        }
-        __syncthreads();
-
-#pragma unroll 1
-        for (int r = 0; r < DD; r++) {
-            if (r == r_lvl0) {
-                for (int idx = tid; idx < a.nlvl0 * kTT; idx += kTreeThreads) {
-                    const int k = idx / kTT, t = idx % kTT;
-                    sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
+    auto run = [&](auto kp) {
+        constexpr int KP = decltype(kp)::value;
+        using PP = PrePass<DD, (KP > 0 ? KP : 1)>;
+        using SK = Sub<(KP > 0 ? KP : 1)>;
+        // register histories of the pre-passes, in the LDS history layout when saved:
+        // pass r, global pair p = group * 2^(KP-1) + q, slot k at hist_off(NP, r) + p*2^r + k
+        float hreg[PP::GPW][SK::NH];
+        if constexpr (KP > 0) {
+#pragma unroll
+            for (int gg = 0; gg < PP::GPW; gg++) {
+                const int grp = wid + gg * PP::NWV;
+#pragma unroll
+                for (int r = 0; r < KP; r++) {
+                    const int n = 1 << r;
+#pragma unroll
+                    for (int q = 0; q < SK::NP; q++) {
+                        const int p = grp * SK::NP + q;
+                        hreg[gg][r * SK::NP + q] = (grp < PP::NG && lane >= 64 - n)
+                                                   ? hist[hist_off(TD::NP, r) + p * n + lane - (64 - n)] : 0.0f;
+                    }
                 }
             }
-            sink.stag_rounds(r);
-            tree_pass<DD>(r, tile, hist, a.main_emit, sink, nv);
         }
-        if (r_lvl0 == DD) {
-            for (int idx = tid; idx < a.nlvl0 * kTT; idx += kTreeThreads) {
-                const int k = idx / kTT, t = idx % kTT;
-                sink.emit(a.lvl0[2 * k + 1], t, tile[a.lvl0[2 * k] * kTT + t]);
+
+        for (int t0 = 0; t0 < Tp; t0 += kTT) {
+            const int nv = min(kTT, Tp - t0);
+            sink.t0 = t0;
+            sink.nv = nv;
+            const int tl = t0 + lane;
+            if constexpr (KP > 0) {
+                const bool live = tl < Tp;
+#pragma unroll
+                for (int gg = 0; gg < PP::GPW; gg++) {
+                    const int grp = wid + gg * PP::NWV;
+                    if (grp < PP::NG) {
+                        float v[SK::NR];
+#pragma unroll
+                        for (int l = 0; l < SK::NR; l++) {
+                            const int4 d = desc[grp * SK::NR + l];
+                            v[l] = live ? ld_st(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : 0.0f;
+                        }
+                        sub_passes<KP>(v, hreg[gg], lane);
+#pragma unroll
+                        for (int l = 0; l < SK::NR; l++)
+                            tile[(grp * SK::NR + l) * kTT + lane] = v[l];
+                    }
+                }
+            } else {
+                for (int fc = wid; fc < TD::D; fc += kTreeThreads / 64) {
+                    const int4 d = desc[fc];
+                    tile[fc * kTT + lane] = ld_st(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0)));
+                }
+            }
+            __syncthreads();
+
+#pragma unroll 1
+            for (int r = KP; r < DD; r++) {
+                if (r == r_lvl0) {
+                    for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
+                        const int k = idx / kTT, t = idx % kTT;
+                        sink.emit(lvl0_tab[2 * k + 1], t, tile[lvl0_tab[2 * k] * kTT + t]);
+                    }
+                }
+                sink.stag_rounds(r);
+                tree_pass<DD>(r, tile, hist, main_emit, sink, nv);
+            }
+            if (r_lvl0 == DD) {
+                for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
+                    const int k = idx / kTT, t = idx % kTT;
+                    sink.emit(lvl0_tab[2 * k + 1], t, tile[lvl0_tab[2 * k] * kTT + t]);
+                }
+                __syncthreads();
+            }
+        }
+
+        if constexpr (KP > 0) {
+            const int hi = (Tp & 63) ? (Tp & 63) : 64;
+#pragma unroll
+            for (int gg = 0; gg < PP::GPW; gg++) {
+                const int grp = wid + gg * PP::NWV;
+#pragma unroll
+                for (int r = 0; r < KP; r++) {
+                    const int n = 1 << r;
+#pragma unroll
+                    for (int q = 0; q < SK::NP; q++) {
+                        const int p = grp * SK::NP + q;
+                        if (grp < PP::NG && lane >= hi - n && lane < hi)
+                            hist[hist_off(TD::NP, r) + p * n + lane - (hi - n)] = hreg[gg][r * SK::NP + q];
+                    }
+                }
             }
             __syncthreads();
         }
+    };
+
+    // Pre-passes: up to 4 passes, all before the first subband emission (pass DD - pf_rank).
+    const int kpre = min(min(DD - a.pf_rank, 4), DD);
+    switch (kpre) {
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, (DD >= 2 ? 2 : 1)>{}); break;
+        case 3: run(std::integral_constant<int, (DD >= 3 ? 3 : 1)>{}); break;
+        case 4: run(std::integral_constant<int, (DD >= 4 ? 4 : 1)>{}); break;
+        default: run(std::integral_constant<int, 0>{}); break;
     }
 
     for (int i = tid; i < nst; i += kTreeThreads)

@@ -30,10 +30,13 @@
 
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "tree_regs.hpp"
 
 namespace pirate_amd {
 
 namespace {
+
+using namespace treg;
 
 constexpr int cmin(int a, int b) { return (a < b) ? a : b; }
 
@@ -41,26 +44,6 @@ constexpr int cmin(int a, int b) { return (a < b) ? a : b; }
 // phase B. Measured on chord16k (stage2_t0, fp32 / fp16 ms): 0 rows 1.20 / 1.61, 4 rows
 // 1.07 / 1.46, 8 rows 1.07 / 1.52, 12 rows 1.14 / 1.60 (spills in the tile loop), 16 rows 1.31.
 constexpr int kStage2Rank7Prefetch = 4;
-
-__host__ __device__ constexpr int brev_c(int x, int n)
-{
-    int r = 0;
-    for (int i = 0; i < n; i++)
-        if ((x >> i) & 1)
-            r |= 1 << (n - 1 - i);
-    return r;
-}
-
-// Rank-K sub-tree held in registers.
-template<int K>
-struct Sub {
-    static constexpr int NR = 1 << K;                       // rows
-    static constexpr int NP = (K > 0) ? NR / 2 : 0;         // pairs per pass
-    static constexpr int NH = (K > 0) ? K * NP : 1;         // history registers (>= 1)
-    static constexpr int NST = NP * (NR - 1);               // saved history floats
-    // saved history of (pass r, pair q): the last 2^r samples
-    static constexpr int hoff(int r, int q) { return NP * ((1 << r) - 1) + q * (1 << r); }
-};
 
 template<int DD>
 struct FastDims {
@@ -123,70 +106,6 @@ struct FastDims {
         return T2 * sb + brev_c(j, K1) * (h * (NRB - 1) - h * (h - 1) / 2);
     }
 };
-
-// Stream value at time t - L (L = 0..64): cur at lane t - L, or prev (the previous tile)
-// at lane 64 + t - L when t < L. L is a compile-time constant after unrolling; L = 1 (about
-// half of all shifts) is a whole-wave rotate by one lane in the VALU (DPP wave_ror:1, lane t
-// reads lane t - 1 mod 64), the other lags a ds_bpermute through the LDS crossbar.
-__device__ __forceinline__ float lane_shift(float cur, float prev, int L, int lane)
-{
-    const float w = (lane >= 64 - L) ? prev : cur;
-    if (L == 1)
-        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(w), 0x13C, 0xf, 0xf, false));
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane - L) & 63) << 2, __float_as_int(w)));
-}
-
-template<int K>
-__device__ __forceinline__ void sub_passes(float (&x)[Sub<K>::NR], float (&h)[Sub<K>::NH], int lane)
-{
-    constexpr int NP = Sub<K>::NP;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            const int fout = q >> r, dmb = q & ((1 << r) - 1);
-            const int e = (fout << (r + 1)) + dmb, o = e + (1 << r);
-            const int L = brev_c(dmb, r);
-            const float xe = x[e], xo = x[o], hp = h[r * NP + q];
-            const float a0 = (L == 0) ? xe : lane_shift(xe, hp, L, lane);
-            const float a1 = lane_shift(xe, hp, L + 1, lane);
-            h[r * NP + q] = xe;
-            x[e] = kRsqrt2 * (a0 + xo);
-            x[o] = kRsqrt2 * (a1 + xo);
-        }
-    }
-}
-
-template<int K>
-__device__ __forceinline__ void load_hist(float (&h)[Sub<K>::NH], const float *src, int lane)
-{
-    constexpr int NP = Sub<K>::NP;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const int n = 1 << r;
-#pragma unroll
-        for (int q = 0; q < NP; q++)
-            h[r * NP + q] = (lane >= 64 - n) ? src[Sub<K>::hoff(r, q) + lane - (64 - n)] : 0.0f;
-    }
-    if constexpr (K == 0)
-        h[0] = 0.0f;
-}
-
-// Saves the last 2^r samples of each pass history; 'hi' = valid lanes of the last tile
-// (64, or 32 for a half tile).
-template<int K>
-__device__ __forceinline__ void save_hist(const float (&h)[Sub<K>::NH], float *dst, int lane, int hi)
-{
-    constexpr int NP = Sub<K>::NP;
-#pragma unroll
-    for (int r = 0; r < K; r++) {
-        const int n = 1 << r;
-#pragma unroll
-        for (int q = 0; q < NP; q++)
-            if (lane >= hi - n && lane < hi)
-                dst[Sub<K>::hoff(r, q) + lane - (hi - n)] = h[r * NP + q];
-    }
-}
 
 // Per-wave view of the phase-B delay lines of sub-tree j.
 template<int DD>
