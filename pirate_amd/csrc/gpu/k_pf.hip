@@ -22,6 +22,14 @@
 // tiles are aligned to output bins. Positions before the row start are read as zeros,
 // which equals the zero initial state of the windows.
 //
+// The chunk's time axis is split into nseg segments (blockIdx.z), one workgroup each, so
+// that short chunks still fill the chip. A segment's lanes start from zero windows and run
+// the tpad positions before it unevaluated: every window entry at the segment start is a
+// sum over at most the last tpad samples (level l holds boxcar sums of 2^l samples ending at
+// most 3*2^l positions back, and 4*2^l <= 2*Wmax <= tpad), so the windows, and hence every
+// candidate, equal those of one lane walking the whole chunk. The running variance sums
+// depend on the order of the bins, so they use one segment.
+//
 // Rows are [tpad history | Tp samples | future (final lags)]; at the end of a chunk the
 // history and future regions are rotated into place for the next chunk.
 
@@ -236,6 +244,8 @@ k_pf(PfArgs a)
     const int ndpf = kPfThreads / M;
     const int dpf0 = blockIdx.x * ndpf;
     const int b = blockIdx.y;
+    const int ntseg = a.Tp / (TL * a.nseg);      // evaluated tiles of this segment
+    const int tile0 = blockIdx.z * ntseg;        // its first tile in the chunk
     const int nd = min(ndpf, a.ndm_out - dpf0);
     const int nstreams = nd * M;
     const bool active = tid < nstreams;
@@ -263,11 +273,11 @@ k_pf(PfArgs a)
     }
 
     const int nwarm = (a.tpad + TL - 1) / TL;
-    const int ntiles = nwarm + a.Tp / TL;
+    const int ntiles = nwarm + ntseg;
     const float *xs = tile + tid * STR;
 
     for (int it = 0; it < ntiles; it++) {
-        const int p0 = a.tpad + (it - nwarm) * TL;
+        const int p0 = a.tpad + (tile0 + it - nwarm) * TL;
 
         // ---- stage [nstreams][TL] samples (4 per load, VPR loads per row segment)
 #pragma unroll
@@ -294,7 +304,7 @@ k_pf(PfArgs a)
                     ln.template bin<false>(xs + j * S::DOUT);
             }
         } else {
-            const int tb0 = (it - nwarm) * NB;
+            const int tb0 = (tile0 + it - nwarm) * NB;
             if (active) {
 #pragma unroll 1
                 for (int j = 0; j < NB; j++) {
@@ -316,7 +326,7 @@ k_pf(PfArgs a)
 
         // ---- reduce over multiplets: first (level, m) in loop order with the largest value
         if (it >= nwarm) {
-            const int tb0 = (it - nwarm) * NB;
+            const int tb0 = (tile0 + it - nwarm) * NB;
             for (int idx = tid; idx < nd * NB; idx += kPfThreads) {
                 const int d = idx / NB, j = idx - d * NB;
                 const int s0 = d * M;
@@ -354,7 +364,11 @@ k_pf(PfArgs a)
     // Done in column blocks of width min(Tp, tpad + flmax) in ascending order, with a
     // barrier between blocks: within a block the positions read ([c0 + Tp, ...)) and
     // written ([c0, c0 + width)) are disjoint, and a later block reads only positions at or
-    // above every position an earlier block wrote.
+    // above every position an earlier block wrote. Segment 0 rotates; the host chooses
+    // segments of at least nwarm*TL + flmax positions, so no other segment reads below
+    // tpad + flmax, the positions written here.
+    if (blockIdx.z != 0)
+        return;
     const int nrot = a.tpad + a.flmax;
     const int wcol = min(a.Tp, nrot);
     for (int c0 = 0; c0 < nrot; c0 += wcol) {
@@ -388,11 +402,35 @@ static void launch_pf_lw(const PfArgs &a, hipStream_t st, dim3 grid)
     }
 }
 
-template<typename Tst>
-static void launch_pf_t(const PfArgs &a, hipStream_t st)
+// Time segments per chunk: the fewest (a power of two) that give at least kPfTargetBlocks
+// workgroups, within segments of whole tiles of at least nwarm*TL + flmax positions (see
+// the rotation in k_pf). With out_var, one segment.
+static constexpr long kPfTargetBlocks = 2048;
+
+static int pf_nseg(const PfArgs &a, long nblocks)
 {
+    if (a.out_var)
+        return 1;
+    const int TL = (a.Dout >= 4) ? 32 : 8 * a.Dout;
+    const int nwarm = (a.tpad + TL - 1) / TL;
+    int nseg = 1;
+    while (nblocks * nseg < kPfTargetBlocks) {
+        const int n2 = 2 * nseg;
+        if (a.Tp % (TL * n2) != 0 || a.Tp / n2 < nwarm * TL + a.flmax)
+            break;
+        nseg = n2;
+    }
+    return nseg;
+}
+
+template<typename Tst>
+static void launch_pf_t(const PfArgs &a0, hipStream_t st)
+{
+    PfArgs a = a0;
     const int ndpf = kPfThreads / a.M;
-    dim3 grid((a.ndm_out + ndpf - 1) / ndpf, a.B);
+    const unsigned nbx = unsigned((a.ndm_out + ndpf - 1) / ndpf);
+    a.nseg = pf_nseg(a, long(nbx) * a.B);
+    dim3 grid{nbx, unsigned(a.B), unsigned(a.nseg)};
     switch ((a.P - 1) / 3) {
         case 0: launch_pf_lw<Tst, 0>(a, st, grid); break;
         case 1: launch_pf_lw<Tst, 1>(a, st, grid); break;

@@ -124,6 +124,7 @@ struct PfArgs {
     long out_bstride;
     int B, ndm_out, M, N, P, nlev, Tp, Dout, Dcore, ndm_wt, nt_wt, nt_out;
     int beam0;                // global index of the batch's first beam (sb rows are per global beam)
+    int nseg = 1;             // time segments per chunk (set by launch_pf)
     double *out_var = nullptr;   // optional (B, ndm_out, M, P) float64: per-chunk mean square of each
                                  // (multiplet, profile)'s weighted candidates
 };
