@@ -56,6 +56,9 @@ GpuDedisperser::~GpuDedisperser()
     for (hipStream_t x : {compute_stream, d2h_stream, h2d_stream})
         if (x)
             (void) hipStreamSynchronize(x);
+    for (hipStream_t x : branch_streams)
+        if (x && x != compute_stream)
+            (void) hipStreamSynchronize(x);
     auto freeit = [](DeviceArray &a) { if (a.ptr) (void) hipFree(a.ptr); a.ptr = nullptr; };
     freeit(input_slots);
     freeit(grid_buf);
@@ -79,8 +82,14 @@ GpuDedisperser::~GpuDedisperser()
     for (hipStream_t x : {d2h_stream, h2d_stream})
         if (x)
             (void) hipStreamDestroy(x);
+    for (size_t p = 1; p < branch_streams.size(); p++)
+        if (branch_streams[p])
+            (void) hipStreamDestroy(branch_streams[p]);
+    for (hipEvent_t e : {ev_fork, ev_split_free})
+        if (e)
+            (void) hipEventDestroy(e);
     for (auto *v : {&ev_input_free, &ev_input_ready, &ev_output_ready, &ev_output_released, &timing_event_pool,
-                    &ev_spill_s1, &ev_spill_d2h, &ev_spill_h2d, &ev_spill_scatter})
+                    &ev_spill_s1, &ev_spill_d2h, &ev_spill_h2d, &ev_spill_scatter, &ev_join})
         for (hipEvent_t e : *v)
             (void) hipEventDestroy(e);
     if (compute_stream)
@@ -509,6 +518,12 @@ void GpuDedisperser::allocate()
         HIP_CHECK(hipStreamCreateWithFlags(&d2h_stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&h2d_stream, hipStreamNonBlocking));
     }
+    branch_streams.assign(npri, nullptr);
+    for (long p = 1; p < npri; p++)
+        HIP_CHECK(hipStreamCreateWithFlags(&branch_streams[p], hipStreamNonBlocking));
+    mkev(ev_join, npri);
+    HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_split_free, hipEventDisableTiming));
     mkev(ev_input_free, nstreams);
     mkev(ev_input_ready, nstreams);
     mkev(ev_output_ready, nbatches_out);
@@ -560,7 +575,20 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
     for (long p = 0; p < npri; p++)
         if (nspill[p] > 0)
             spill_h2d(p, ichunk, ibatch, beam0);
+    // Primary tree p's stage 1, spill scatter, stage-2 trees and peak finders run on branch
+    // stream p (0: st); st waits for every branch at the end. The branches p >= 1 start
+    // after the gridding, or, when tree 0's stage 2 is the register-tree kernel, after it:
+    // stage 1 and that stage 2 run near the HBM bandwidth on their own (chord16k fp32:
+    // 5.57 ms/step serial, 5.66 with the branches from the gridding on), while the
+    // peak finders and the subband stage-2 kernels are bound by VALU or LDS latency and
+    // leave bandwidth to share (chime_sb1-shaped, fp16: 15.68 -> 15.11 ms).
+    const bool late_fork = tables[0].fast;
+    if (!late_fork)
+        HIP_CHECK(hipEventRecord(ev_fork, st));
     for (long p = 0; p < npri; p++) {
+        hipStream_t sp = p ? branch_streams[p] : st;
+        if (p)
+            HIP_CHECK(hipStreamWaitEvent(sp, ev_fork, 0));
         Stage1Args s;
         const long Tp = nt_in >> p;
         s.in = p ? lds_out[p].ptr : grid_buf.ptr;
@@ -579,96 +607,106 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.chunk = ichunk;
         hipEvent_t ev;
         const std::string nm = "stage1_p" + std::to_string(p);
-        timed_begin(nm, st, ev);
+        timed_begin(nm, sp, ev);
         if (nspill[p] > 0)     // the outbox is free once the previous chunk's copy has read it
-            HIP_CHECK(hipStreamWaitEvent(st, ev_spill_d2h[p * nbatches + ibatch], 0));
-        launch_stage1_fast(dtype, s, st);
-        timed_end(nm, st, ev);
-        if (nspill[p] > 0)
-            spill_d2h(p, ichunk, ibatch, beam0, st);
-    }
-    for (long p = 0; p < npri; p++)
-        if (nspill[p] > 0)
-            spill_scatter(p, ichunk, ibatch, beam0, st);
-    for (long it = 0; it < ntrees; it++) {
-        const DedispersionTree &t = plan->trees[it];
-        const TreeTables &T = tables[it];
-        const long p = t.primary_tree_index;
-        Stage2Args s;
-        s.ring.base = rings[p].ptr;
-        s.ring.row_off = (const long *) ring_row_off[p].ptr;
-        s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
-        s.ring.bstride = ring_bstride[p];
-        s.row_lag = (const int *) stage2_lags[it].ptr;
-        s.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
-        s.sb_bstride = t.ndm_out * T.M * T.rowlen;
-        s.rowlen = T.rowlen;
-        s.tpad = T.tpad;
-        s.main_emit = (const int *) d_main_emit[it].ptr;
-        s.stag = (const int *) d_stag[it].ptr;
-        s.lvl0 = (const int *) d_lvl0[it].ptr;
-        s.fl = (const int *) d_fl[it].ptr;
-        s.nstag = T.nstag;
-        s.nlvl0 = T.nlvl0;
-        s.stag_hist_nelts = T.split ? T.stag_hist_g : T.stag_hist_nelts;
-        s.stag_goff = (const int *) d_stag_goff[it].ptr;
-        s.stag_begin = (const int *) d_stag_begin[it].ptr;
-        s.M = T.M;
-        s.pf_rank = int(t.frequency_subbands.pf_rank);
-        s.state = (float *) s2_state[it].ptr;
-        s.state_bstride = s2_state_bstride[it];
-        s.B = int(B);
-        s.amb = t.amb_rank;
-        s.dd = t.dd_rank;
-        s.Tp = t.nt_ds;
-        s.beam0 = beam0;
-        s.amb1_dd1 = t.amb_rank;
-        s.chunk = ichunk;
-        s.out_stream = (const int *) d_out_stream[it].ptr;
-        hipEvent_t ev;
-        const std::string nm2 = "stage2_t" + std::to_string(it);
-        timed_begin(nm2, st, ev);
-        if (T.fast)
-            launch_stage2_fast(dtype, s, st);
-        else if (T.split)
-            launch_stage2_split(dtype, s, (float *) split_x.ptr, st);
-        else
-            launch_stage2(dtype, s, st);
-        timed_end(nm2, st, ev);
+            HIP_CHECK(hipStreamWaitEvent(sp, ev_spill_d2h[p * nbatches + ibatch], 0));
+        launch_stage1_fast(dtype, s, sp);
+        timed_end(nm, sp, ev);
+        if (nspill[p] > 0) {
+            spill_d2h(p, ichunk, ibatch, beam0, sp);
+            spill_scatter(p, ichunk, ibatch, beam0, sp);
+        }
+        for (long it = 0; it < ntrees; it++) {
+            const DedispersionTree &t = plan->trees[it];
+            const TreeTables &T = tables[it];
+            if (t.primary_tree_index != p)
+                continue;
+            Stage2Args s;
+            s.ring.base = rings[p].ptr;
+            s.ring.row_off = (const long *) ring_row_off[p].ptr;
+            s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
+            s.ring.bstride = ring_bstride[p];
+            s.row_lag = (const int *) stage2_lags[it].ptr;
+            s.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
+            s.sb_bstride = t.ndm_out * T.M * T.rowlen;
+            s.rowlen = T.rowlen;
+            s.tpad = T.tpad;
+            s.main_emit = (const int *) d_main_emit[it].ptr;
+            s.stag = (const int *) d_stag[it].ptr;
+            s.lvl0 = (const int *) d_lvl0[it].ptr;
+            s.fl = (const int *) d_fl[it].ptr;
+            s.nstag = T.nstag;
+            s.nlvl0 = T.nlvl0;
+            s.stag_hist_nelts = T.split ? T.stag_hist_g : T.stag_hist_nelts;
+            s.stag_goff = (const int *) d_stag_goff[it].ptr;
+            s.stag_begin = (const int *) d_stag_begin[it].ptr;
+            s.M = T.M;
+            s.pf_rank = int(t.frequency_subbands.pf_rank);
+            s.state = (float *) s2_state[it].ptr;
+            s.state_bstride = s2_state_bstride[it];
+            s.B = int(B);
+            s.amb = t.amb_rank;
+            s.dd = t.dd_rank;
+            s.Tp = t.nt_ds;
+            s.beam0 = beam0;
+            s.amb1_dd1 = t.amb_rank;
+            s.chunk = ichunk;
+            s.out_stream = (const int *) d_out_stream[it].ptr;
+            hipEvent_t ev;
+            const std::string nm2 = "stage2_t" + std::to_string(it);
+            timed_begin(nm2, sp, ev);
+            if (T.fast)
+                launch_stage2_fast(dtype, s, sp);
+            else if (T.split) {
+                // the float32 scratch is shared by every rank-8 split tree
+                HIP_CHECK(hipStreamWaitEvent(sp, ev_split_free, 0));
+                launch_stage2_split(dtype, s, (float *) split_x.ptr, sp);
+                HIP_CHECK(hipEventRecord(ev_split_free, sp));
+            }
+            else
+                launch_stage2(dtype, s, sp);
+            timed_end(nm2, sp, ev);
+            if (late_fork && it == 0)
+                HIP_CHECK(hipEventRecord(ev_fork, st));
 
-        PfArgs f;
-        f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
-        f.sb_bstride = s.sb_bstride;
-        f.rowlen = T.rowlen;
-        f.tpad = T.tpad;
-        f.flmax = T.flmax;
-        f.m_to_n = (const int *) d_m_to_n[it].ptr;
-        const long wslot_n = B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
-        f.wt = (const float *) weights[it].ptr + wslot * wslot_n;
-        f.wt_bstride = t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
-        const long oslot_n = B * t.ndm_out * t.nt_out;
-        f.out_max = (char *) out_max[it].ptr + oslot * oslot_n * elt_bytes;
-        f.out_argmax = (unsigned int *) out_argmax[it].ptr + oslot * oslot_n;
-        f.out_bstride = t.ndm_out * t.nt_out;
-        f.B = int(B);
-        f.ndm_out = int(t.ndm_out);
-        f.M = T.M;
-        f.N = T.N;
-        f.P = int(t.nprofiles);
-        f.nlev = T.nlev;
-        f.Tp = t.nt_ds;
-        f.Dout = int(t.pf.time_downsampling);
-        f.Dcore = int(t.Dcore);
-        f.ndm_wt = int(t.ndm_wt);
-        f.nt_wt = int(t.nt_wt);
-        f.nt_out = int(t.nt_out);
-        f.beam0 = beam0;
-        if (params.enable_variances)
-            f.out_var = (double *) out_var[it].ptr + oslot * B * t.ndm_out * T.M * t.nprofiles;
-        const std::string nmp = "pf_t" + std::to_string(it);
-        timed_begin(nmp, st, ev);
-        launch_pf(dtype, f, st);
-        timed_end(nmp, st, ev);
+            PfArgs f;
+            f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
+            f.sb_bstride = s.sb_bstride;
+            f.rowlen = T.rowlen;
+            f.tpad = T.tpad;
+            f.flmax = T.flmax;
+            f.m_to_n = (const int *) d_m_to_n[it].ptr;
+            const long wslot_n = B * t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
+            f.wt = (const float *) weights[it].ptr + wslot * wslot_n;
+            f.wt_bstride = t.ndm_wt * t.nt_wt * t.nprofiles * T.N;
+            const long oslot_n = B * t.ndm_out * t.nt_out;
+            f.out_max = (char *) out_max[it].ptr + oslot * oslot_n * elt_bytes;
+            f.out_argmax = (unsigned int *) out_argmax[it].ptr + oslot * oslot_n;
+            f.out_bstride = t.ndm_out * t.nt_out;
+            f.B = int(B);
+            f.ndm_out = int(t.ndm_out);
+            f.M = T.M;
+            f.N = T.N;
+            f.P = int(t.nprofiles);
+            f.nlev = T.nlev;
+            f.Tp = t.nt_ds;
+            f.Dout = int(t.pf.time_downsampling);
+            f.Dcore = int(t.Dcore);
+            f.ndm_wt = int(t.ndm_wt);
+            f.nt_wt = int(t.nt_wt);
+            f.nt_out = int(t.nt_out);
+            f.beam0 = beam0;
+            if (params.enable_variances)
+                f.out_var = (double *) out_var[it].ptr + oslot * B * t.ndm_out * T.M * t.nprofiles;
+            const std::string nmp = "pf_t" + std::to_string(it);
+            timed_begin(nmp, sp, ev);
+            launch_pf(dtype, f, sp);
+            timed_end(nmp, sp, ev);
+        }
+        if (p) {
+            HIP_CHECK(hipEventRecord(ev_join[p], sp));
+            HIP_CHECK(hipStreamWaitEvent(st, ev_join[p], 0));
+        }
     }
     HIP_CHECK(hipGetLastError());
 }

@@ -103,6 +103,10 @@ public:
     std::string kernel_timing_report();
 
     hipStream_t compute_stream = nullptr;
+    // Primary trees p >= 1 run their stage 1, stage 2 and peak finding on branch_streams[p]
+    // (branch 0 is the calling stream), forked after the gridding and joined before the chain
+    // returns, so that the primary trees' kernels overlap.
+    std::vector<hipStream_t> branch_streams;
 
     // Per-tree stage-2 tables (host copies, exposed for tests).
     struct TreeTables {
@@ -147,6 +151,8 @@ private:
     std::vector<long> nspill, outbox_off, inbox_off, host_bstride;   // [p]
     hipStream_t d2h_stream = nullptr, h2d_stream = nullptr;
     std::vector<hipEvent_t> ev_spill_s1, ev_spill_d2h, ev_spill_h2d, ev_spill_scatter;   // [p * nbatches + batch]
+    hipEvent_t ev_fork = nullptr, ev_split_free = nullptr;
+    std::vector<hipEvent_t> ev_join;                          // [p]
     std::vector<DeviceArray> spill_rows;                     // [p] device int: ring row of each outbox row
     std::vector<DeviceArray> ring1_row_off, ring1_row_cap;   // [p] stage-1 (write) side
     std::vector<void *> host_ring;                           // [p] pinned, (total_beams, host_bstride)
