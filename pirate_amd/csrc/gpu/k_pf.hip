@@ -41,6 +41,14 @@ namespace pirate_amd {
 static constexpr int kPfThreads = 256;
 static constexpr int kPfMaxDout = 32;
 
+// Staging tile length (positions) for output time downsampling Dout: whole output bins, at
+// least kPfTileMin positions when Dout >= 4, 8 bins below.
+static constexpr int kPfTileMin = 16;
+__host__ __device__ constexpr int pf_tile_len(int dout)
+{
+    return (dout >= 4) ? ((dout > kPfTileMin) ? dout : kPfTileMin) : 8 * dout;
+}
+
 bool pf_config_supported(int M, int tpad, int nlev, int Dcore, int Dout, int Tp)
 {
     (void) nlev;
@@ -59,7 +67,7 @@ struct PfShape {
     static constexpr int P = 1 + 3 * LW;
     static constexpr int DOUT = 1 << LD;
     static constexpr int DCORE = (DOUT < 8) ? DOUT : 8;
-    static constexpr int TL = (DOUT >= 4) ? 32 : 8 * DOUT;       // tile length (positions)
+    static constexpr int TL = pf_tile_len(DOUT);                 // tile length (positions)
     static constexpr int NB = TL / DOUT;                          // output bins per tile
     static constexpr int dt(int l) { return ((1 << l) < DCORE) ? (1 << l) : DCORE; }
     static constexpr int sgrid(int l) { return (1 << l) / dt(l); }   // grid points per 2^l samples
@@ -411,7 +419,7 @@ static int pf_nseg(const PfArgs &a, long nblocks)
 {
     if (a.out_var)
         return 1;
-    const int TL = (a.Dout >= 4) ? 32 : 8 * a.Dout;
+    const int TL = pf_tile_len(a.Dout);
     const int nwarm = (a.tpad + TL - 1) / TL;
     int nseg = 1;
     while (nblocks * nseg < kPfTargetBlocks) {

@@ -5,9 +5,10 @@ Usage (after profiling bench.py, see --help):
     python -m pirate_amd.profiling --stats DIR [--fetch DIR --write DIR] --md OUT.md --json OUT.json
 
 DIR is a rocprofv3 output directory written with ``--output-format csv -o run``. Kernel
-dispatches of this library are named by their place in the chain (grid_lds,
-stage1_p<p>, stage2_t<i>, pf_t<i>), in launch order: one chain per step, so the i-th
-dispatch of the library within a step has a fixed name. HBM counters are scaled by what the
+dispatches of this library are named by their place in the chain, in host launch order
+(dispatch ids): grid_lds, then per primary tree p its stage1_p<p> followed by stage2_t<i>,
+pf_t<i> of each tree i of p. One chain per step, so the i-th dispatch of the library within
+a step has a fixed name, whichever stream it ran on. HBM counters are scaled by what the
 same counter reads on a reference copy of known size (the 1 GiB copy that bench.py makes
 with --calib-copy): bytes = counter * (2^30 / counter_of_copy).
 """
@@ -36,11 +37,27 @@ def _short(name):
     return s.split('(')[0]
 
 
-def chain_names(npri, ntrees):
-    names = ['grid_lds'] + ['stage1_p%d' % p for p in range(npri)]
-    for i in range(ntrees):
-        names += ['stage2_t%d' % i, 'pf_t%d' % i]
+def chain_names(npri, ntrees, tree_primary=None):
+    """Chain kernel names in launch order; tree_primary[i] = primary tree of tree i (default:
+    tree i of primary tree i, which needs ntrees == npri)."""
+    if tree_primary is None:
+        if ntrees != npri:
+            raise ValueError('give tree_primary when trees and primary trees differ in number')
+        tree_primary = list(range(ntrees))
+    names = ['grid_lds']
+    for p in range(npri):
+        names.append('stage1_p%d' % p)
+        for i in range(ntrees):
+            if tree_primary[i] == p:
+                names += ['stage2_t%d' % i, 'pf_t%d' % i]
     return names
+
+
+def _launch_key(r):
+    for k in ('Dispatch_Id', 'Correlation_Id'):
+        if r.get(k) not in (None, ''):
+            return int(r[k])
+    return int(r['Start_Timestamp'])
 
 
 def label_dispatches(kernel_names, names):
@@ -60,7 +77,7 @@ def label_dispatches(kernel_names, names):
 
 def kernel_times(stats_dir, names):
     tr = _rows(os.path.join(stats_dir, 'run_kernel_trace.csv'))
-    tr.sort(key=lambda r: int(r['Start_Timestamp']))
+    tr.sort(key=_launch_key)
     labels = label_dispatches([r['Kernel_Name'] for r in tr], names)
     acc = collections.defaultdict(list)
     kname = {}
@@ -104,6 +121,8 @@ def main(argv=None):
     ap.add_argument('--write')
     ap.add_argument('--npri', type=int, default=4)
     ap.add_argument('--ntrees', type=int, default=4)
+    ap.add_argument('--tree-primary', help='comma-separated primary tree of each tree (default: i -> i)')
+    ap.add_argument('--note', action='append', default=[], help='paragraph to add under the commands')
     ap.add_argument('--alg-json', help='bench.py JSON line (per-kernel algorithmic bytes)')
     ap.add_argument('--title', default='rocprofv3 summary')
     ap.add_argument('--command', action='append', default=[], help='command lines to record')
@@ -111,7 +130,8 @@ def main(argv=None):
     ap.add_argument('--json', required=True)
     a = ap.parse_args(argv)
 
-    names = chain_names(a.npri, a.ntrees)
+    tp = [int(x) for x in a.tree_primary.split(',')] if a.tree_primary else None
+    names = chain_names(a.npri, a.ntrees, tp)
     times = kernel_times(a.stats, names)
     alg = {}
     if a.alg_json:
@@ -130,6 +150,8 @@ def main(argv=None):
     lines = ['# ' + a.title, '', 'Commands (run from the repository root on the GPU host):', '']
     lines += ['    ' + c for c in a.command]
     lines += ['']
+    for n in a.note:
+        lines += [n, '']
     if fetch is not None:
         out['fetch_scale'] = fscale
         out['write_scale'] = wscale
