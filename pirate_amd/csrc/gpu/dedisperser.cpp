@@ -621,57 +621,57 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             const TreeTables &T = tables[it];
             if (t.primary_tree_index != p)
                 continue;
-            Stage2Args s;
-            s.ring.base = rings[p].ptr;
-            s.ring.row_off = (const long *) ring_row_off[p].ptr;
-            s.ring.row_cap = (const long *) ring_row_cap[p].ptr;
-            s.ring.bstride = ring_bstride[p];
-            s.row_lag = (const int *) stage2_lags[it].ptr;
-            s.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
-            s.sb_bstride = t.ndm_out * T.M * T.rowlen;
-            s.rowlen = T.rowlen;
-            s.tpad = T.tpad;
-            s.main_emit = (const int *) d_main_emit[it].ptr;
-            s.stag = (const int *) d_stag[it].ptr;
-            s.lvl0 = (const int *) d_lvl0[it].ptr;
-            s.fl = (const int *) d_fl[it].ptr;
-            s.nstag = T.nstag;
-            s.nlvl0 = T.nlvl0;
-            s.stag_hist_nelts = T.split ? T.stag_hist_g : T.stag_hist_nelts;
-            s.stag_goff = (const int *) d_stag_goff[it].ptr;
-            s.stag_begin = (const int *) d_stag_begin[it].ptr;
-            s.M = T.M;
-            s.pf_rank = int(t.frequency_subbands.pf_rank);
-            s.state = (float *) s2_state[it].ptr;
-            s.state_bstride = s2_state_bstride[it];
-            s.B = int(B);
-            s.amb = t.amb_rank;
-            s.dd = t.dd_rank;
-            s.Tp = t.nt_ds;
-            s.beam0 = beam0;
-            s.amb1_dd1 = t.amb_rank;
-            s.chunk = ichunk;
-            s.out_stream = (const int *) d_out_stream[it].ptr;
-            hipEvent_t ev;
+            Stage2Args s2;
+            s2.ring.base = rings[p].ptr;
+            s2.ring.row_off = (const long *) ring_row_off[p].ptr;
+            s2.ring.row_cap = (const long *) ring_row_cap[p].ptr;
+            s2.ring.bstride = ring_bstride[p];
+            s2.row_lag = (const int *) stage2_lags[it].ptr;
+            s2.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
+            s2.sb_bstride = t.ndm_out * T.M * T.rowlen;
+            s2.rowlen = T.rowlen;
+            s2.tpad = T.tpad;
+            s2.main_emit = (const int *) d_main_emit[it].ptr;
+            s2.stag = (const int *) d_stag[it].ptr;
+            s2.lvl0 = (const int *) d_lvl0[it].ptr;
+            s2.fl = (const int *) d_fl[it].ptr;
+            s2.nstag = T.nstag;
+            s2.nlvl0 = T.nlvl0;
+            s2.stag_hist_nelts = T.split ? T.stag_hist_g : T.stag_hist_nelts;
+            s2.stag_goff = (const int *) d_stag_goff[it].ptr;
+            s2.stag_begin = (const int *) d_stag_begin[it].ptr;
+            s2.M = T.M;
+            s2.pf_rank = int(t.frequency_subbands.pf_rank);
+            s2.state = (float *) s2_state[it].ptr;
+            s2.state_bstride = s2_state_bstride[it];
+            s2.B = int(B);
+            s2.amb = t.amb_rank;
+            s2.dd = t.dd_rank;
+            s2.Tp = t.nt_ds;
+            s2.beam0 = beam0;
+            s2.amb1_dd1 = t.amb_rank;
+            s2.chunk = ichunk;
+            s2.out_stream = (const int *) d_out_stream[it].ptr;
+            hipEvent_t ev2;
             const std::string nm2 = "stage2_t" + std::to_string(it);
-            timed_begin(nm2, sp, ev);
+            timed_begin(nm2, sp, ev2);
             if (T.fast)
-                launch_stage2_fast(dtype, s, sp);
+                launch_stage2_fast(dtype, s2, sp);
             else if (T.split) {
                 // the float32 scratch is shared by every rank-8 split tree
                 HIP_CHECK(hipStreamWaitEvent(sp, ev_split_free, 0));
-                launch_stage2_split(dtype, s, (float *) split_x.ptr, sp);
+                launch_stage2_split(dtype, s2, (float *) split_x.ptr, sp);
                 HIP_CHECK(hipEventRecord(ev_split_free, sp));
             }
             else
-                launch_stage2(dtype, s, sp);
-            timed_end(nm2, sp, ev);
+                launch_stage2(dtype, s2, sp);
+            timed_end(nm2, sp, ev2);
             if (late_fork && it == 0)
                 HIP_CHECK(hipEventRecord(ev_fork, st));
 
             PfArgs f;
             f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
-            f.sb_bstride = s.sb_bstride;
+            f.sb_bstride = s2.sb_bstride;
             f.rowlen = T.rowlen;
             f.tpad = T.tpad;
             f.flmax = T.flmax;
@@ -699,9 +699,9 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             if (params.enable_variances)
                 f.out_var = (double *) out_var[it].ptr + oslot * B * t.ndm_out * T.M * t.nprofiles;
             const std::string nmp = "pf_t" + std::to_string(it);
-            timed_begin(nmp, sp, ev);
+            timed_begin(nmp, sp, ev2);
             launch_pf(dtype, f, sp);
-            timed_end(nmp, sp, ev);
+            timed_end(nmp, sp, ev2);
         }
         if (p) {
             HIP_CHECK(hipEventRecord(ev_join[p], sp));
