@@ -50,7 +50,10 @@ struct FastDims {
     static constexpr int D = 1 << DD;
     static constexpr int K1 = (DD + 1) / 2, K2 = DD / 2;
     static constexpr int NRA = 1 << K1, NRB = 1 << K2;
-    static constexpr int NW = cmin(8, cmin(NRA, NRB));     // waves per workgroup
+    // waves per workgroup: rank 8 runs one sub-tree per wave and phase in 16 waves (128
+    // VGPRs each); 8 waves with two sub-trees each (256 VGPRs, 2 waves per SIMD) measured
+    // slower (chime.yml-shaped stage 2, 8 beams, float16: 4.29 vs 3.71 ms)
+    static constexpr int NW = (DD >= 8) ? 16 : cmin(8, cmin(NRA, NRB));
     static constexpr int SA = NRB / NW, SB = NRA / NW;     // sub-trees per wave in phase A / B
     static constexpr int T2 = NRB * (NRB - 1) / 2;
     static constexpr int SUMLAG = T2 * (NRA * (NRA - 1) / 2);
@@ -75,8 +78,8 @@ struct FastDims {
     static constexpr int DL_FLOATS = ((DL_USED + 256 + 3) / 4) * 4;   // + padding for the shift reads
     static constexpr int LDS_BYTES = 4 * DL_FLOATS + 2 * 16 * D;   // + two int4 row-descriptor tables
     // rank 7: 62 KB of LDS allows 2 workgroups (16 waves) per CU, so aim for 128 VGPRs;
-    // rank 8: one workgroup (8 waves, two sub-trees per wave and phase) per CU, 256 VGPRs
-    static constexpr int WAVES_PER_EU = (DD == 7) ? 4 : 2;
+    // rank 8: one workgroup (16 waves) per CU, also 128 VGPRs
+    static constexpr int WAVES_PER_EU = (DD >= 7) ? 4 : 2;
 
     __host__ __device__ static constexpr int lag_c(int h, int j) { return (NRB - 1 - h) * brev_c(j, K1); }
     // phase-A write offset of row (h, j) (h = the wave's sub-tree, j unrolled)
