@@ -199,8 +199,10 @@ struct Stage2Sink {
     }
 };
 
+// 128 VGPRs (4 waves per SIMD; the rank-7 instances spill a few registers at this limit,
+// against 3 waves at 129-134): chime_sb1-shaped chain, 8 beams, float16, 15.15 -> 14.86 ms.
 template<typename Tst, int DD>
-__global__ void __launch_bounds__(kTreeThreads)
+__global__ void __launch_bounds__(kTreeThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_stage2(Stage2Args a)
 {
     using TD = TreeDims<DD>;
@@ -659,7 +661,7 @@ static void launch_stage2_t(const Stage2Args &a, hipStream_t st)
     size_t shm = stage2_lds_bytes(a.dd, a.stag_hist_nelts);
 #define PDD_S2(DDV)                                                                           \
     case DDV: {                                                                               \
-        hipFuncSetAttribute((const void *) k_stage2<Tst, DDV>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)); \
+        (void) hipFuncSetAttribute((const void *) k_stage2<Tst, DDV>, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)); \
         hipLaunchKernelGGL((k_stage2<Tst, DDV>), grid, block, shm, st, a);                    \
         break;                                                                                \
     }
