@@ -108,7 +108,9 @@ int pdd_plan_pf_variances(const pdd_plan *plan, const double *freq_variances, in
 /* ---- GpuDedisperser (include/pirate/Dedisperser.hpp:45-392) ---- */
 
 /* GpuDedisperser::Params, Dedisperser.hpp:47-86 (stream_pool replaced by an internal
- * compute stream; cuda_device_id -> device). Zero means "default" for the counts.
+ * compute stream; cuda_device_id -> device). Zero means "default" for the counts:
+ * nbatches_out and nbatches_wt default to num_active_batches and must not be below it
+ * (GpuDedisperser.cpp:116-123).
  * enable_variances: the peak finder also writes, per output slot and tree, the float64
  * (B, ndm_out, M, nprofiles) mean square of each (multiplet, profile)'s weighted candidates
  * over the chunk -- the out_var of the reference peak finder (ReferenceDedisperserBase::Params::
@@ -188,7 +190,8 @@ int pdd_output_ipc_handle(pdd_dedisperser *dd, void *handle, int64_t *nbytes);
 int pdd_output_array_offset(const pdd_dedisperser *dd, int64_t itree, int which, int64_t *byte_offset);
 
 /* Consumer side (FrbGrouper.cpp:565-581): maps a handle from another process of this node
- * on HIP device 'device'; *base receives its device address. pdd_ipc_close unmaps it. */
+ * on HIP device 'device'; *base receives its device address. pdd_ipc_close waits for the
+ * device's queued work (which may read the mapping), then unmaps it. */
 int pdd_ipc_open(const void *handle, int device, void **base);
 int pdd_ipc_close(void *base, int device);
 
@@ -224,6 +227,76 @@ void pdd_ringbuf_copy_destroy(pdd_ringbuf_copy *k);
  * segment is checked against ringbuf_nbytes before launch. Asynchronous on 'stream'. */
 int pdd_ringbuf_copy_launch(pdd_ringbuf_copy *k, void *ringbuf, int64_t ringbuf_nbytes, int64_t ichunk,
                             int64_t ibatch, void *stream);
+
+/* PCIe bytes per beam batch moved by the host spill of lag-ring rows, in each direction (0
+ * without max_gpu_clag): the memcpy_h2g / memcpy_g2h entries of the reference's
+ * ResourceTracker (include/pirate/ResourceTracker.hpp:14-75, GpuDedisperser.cpp:183-190). */
+int pdd_dedisperser_spill_bytes(const pdd_dedisperser *dd, int64_t *nbytes);
+
+/* ---- launch shapes ---- */
+
+/* Launch shapes that change no output, only speed (no reference counterpart: the reference
+ * fixes its launch shapes in its kernel generators). name: "pf_target_blocks" (the peak
+ * finder splits each chunk into the fewest power-of-two time segments giving at least this
+ * many workgroups; default 2048), "grid_xcd_run" (gridding workgroups per XCD run of adjacent
+ * channel pairs; 0 = launch order; default 8), "fork" (primary trees p >= 1 start after the
+ * gridding = 0, after tree 0's stage 2 = 1, automatic = -1, all kernels in order on one stream
+ * = 2, for per-kernel timing). Takes effect at the next launch. */
+int pdd_dedisperser_set_tuning(pdd_dedisperser *dd, const char *name, int64_t value);
+int pdd_dedisperser_get_tuning(const pdd_dedisperser *dd, const char *name, int64_t *value);
+
+/* ---- heterogeneous beam groups, one HIP graph launch per chunk ---- */
+
+/* GpuDedisperserGroup (no reference counterpart: the reference batches homogeneous beams, its
+ * beams_per_batch / num_active_batches of DedispersionConfig.hpp:81-84 and the launch loop of
+ * GpuDedisperser.cpp:988-1105). Each plan is one beam group (its own channels, tree rank,
+ * primary trees, downsampling levels), all its beams in one batch (beams_per_batch ==
+ * beams_per_gpu), every plan with the same chunk (time_samples_per_chunk, time_sample_ms),
+ * none spilling to the host (no max_gpu_clag). The group owns nslots input slots per member
+ * (the caller fills chunk c+1 while chunk c runs). Per chunk,
+ * every member's chain runs as independent branches of one HIP graph, captured once per
+ * (input slot, output slot, weight slot) phase and replayed; the kernels read the chunk index
+ * from a device counter that each replay advances. use_graph = 0 issues the same launches
+ * one by one. */
+typedef struct pdd_group pdd_group;
+typedef struct {
+    int64_t nslots;          /* input slots per member (0 = 2) */
+    int64_t nbatches_out;    /* output slots per member (0 = 2) */
+    int32_t device;
+    int32_t use_graph;
+} pdd_group_params;
+
+int pdd_group_create(const pdd_plan *const *plans, int64_t nplans, const pdd_group_params *params, pdd_group **out);
+void pdd_group_destroy(pdd_group *g);
+int pdd_group_allocate(pdd_group *g);
+int pdd_group_gpu_bytes(const pdd_group *g, int64_t *nbytes);
+
+/* A new handle to member imember's dedisperser (release it with pdd_dedisperser_destroy; the
+ * group keeps the dedisperser alive). Use it for weights (pdd_fill_all_weights,
+ * pdd_fill_analytic_weights), timing and launch shapes; its acquire/release calls are refused. */
+int pdd_group_member(pdd_group *g, int64_t imember, pdd_dedisperser **dd);
+
+/* The chunk protocol: chunks 0, 1, 2, ... in order; acquire_input / release_input_and_launch
+ * alternate, as do acquire_output / release_output. slots[m]: member m's (B_m, F_m, T) input
+ * slot (device, storage dtype), writable on 'stream' after the call. release_input_and_launch
+ * blocks until output ichunk - nbatches_out has been released. acquire_output fills
+ * out_max[k] / out_argmax[k] for k running over (member, tree) in order, each
+ * (B_m, ndm_out, nt_out); they are ready on 'stream' (and on the host with sync != 0) and
+ * valid until release_output. A failing call stops the group; later calls fail with the first
+ * error. */
+int pdd_group_acquire_input(pdd_group *g, int64_t ichunk, void *stream, void **slots, int64_t nmembers);
+int pdd_group_release_input_and_launch(pdd_group *g, int64_t ichunk, void *stream);
+int pdd_group_acquire_output(pdd_group *g, int64_t ichunk, void *stream, int sync, void **out_max,
+                             void **out_argmax, int64_t n);
+int pdd_group_release_output(pdd_group *g, int64_t ichunk, void *stream);
+int pdd_group_stop(pdd_group *g, const char *reason);
+
+/* Graph or one-by-one launches, and a member's launch shapes (see pdd_dedisperser_set_tuning);
+ * both drop the captured graphs, which are re-captured at the next launch. */
+int pdd_group_set_use_graph(pdd_group *g, int use_graph);
+int pdd_group_set_tuning(pdd_group *g, int64_t imember, const char *name, int64_t value);
+/* Nodes of the latest captured chunk graph (0 before the first graph launch). */
+int pdd_group_graph_nodes(const pdd_group *g, int64_t *n);
 
 /* Kernel timing (reference counterpart: the per-kernel KernelTimer loops and ResourceTracker
  * rates of GpuDedisperser::time, src_lib/GpuDedisperser.cpp:1573-1690). When enabled, HIP

@@ -7,10 +7,11 @@ over the C-ABI of libpirate_amd.so (include/pirate_amd.h). Build the library wit
 
 from .config import DedispersionConfig, DedispersionPlan, DedispersionTree, PrimaryTree
 from .dedisperser import GpuDedisperser, GpuDedisperserOutputs, OutputRingView
+from .group import GpuDedisperserGroup
 from .kernels import GpuDequantizationKernel, GpuRingbufCopyKernel
 from .pf_variance import PfAvarApproximation
 from .offline import AssembledFrame, OfflineDedisperser, XEngineMetadata
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
-           'GpuDedisperser', 'GpuDedisperserOutputs', 'OutputRingView', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
+           'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDedisperserGroup', 'OutputRingView', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
            'PfAvarApproximation', 'OfflineDedisperser', 'AssembledFrame', 'XEngineMetadata']

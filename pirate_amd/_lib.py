@@ -24,6 +24,11 @@ class TreeInfo(ctypes.Structure):
         [('dm_min', ctypes.c_double), ('dm_max', ctypes.c_double), ('trigger_frequency', ctypes.c_double)]
 
 
+class GroupParams(ctypes.Structure):
+    _fields_ = [('nslots', ctypes.c_int64), ('nbatches_out', ctypes.c_int64), ('device', ctypes.c_int32),
+                ('use_graph', ctypes.c_int32)]
+
+
 class DedisperserParams(ctypes.Structure):
     _fields_ = [('nbatches_out', ctypes.c_int64), ('nbatches_wt', ctypes.c_int64),
                 ('num_consumers', ctypes.c_int64), ('device', ctypes.c_int32),
@@ -77,6 +82,22 @@ _SIGS = {
     'pdd_kernel_timing': (ctypes.c_int, [vp, ctypes.c_int]),
     'pdd_kernel_timing_report': (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     'pdd_debug_copy': (ctypes.c_int, [vp, ctypes.c_char_p, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    'pdd_dedisperser_spill_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+    'pdd_dedisperser_set_tuning': (ctypes.c_int, [vp, ctypes.c_char_p, i64]),
+    'pdd_dedisperser_get_tuning': (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(i64)]),
+    'pdd_group_create': (ctypes.c_int, [ctypes.POINTER(vp), i64, vp, ctypes.POINTER(vp)]),
+    'pdd_group_destroy': (None, [vp]),
+    'pdd_group_allocate': (ctypes.c_int, [vp]),
+    'pdd_group_gpu_bytes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+    'pdd_group_member': (ctypes.c_int, [vp, i64, ctypes.POINTER(vp)]),
+    'pdd_group_acquire_input': (ctypes.c_int, [vp, i64, vp, ctypes.POINTER(vp), i64]),
+    'pdd_group_release_input_and_launch': (ctypes.c_int, [vp, i64, vp]),
+    'pdd_group_acquire_output': (ctypes.c_int, [vp, i64, vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), i64]),
+    'pdd_group_release_output': (ctypes.c_int, [vp, i64, vp]),
+    'pdd_group_stop': (ctypes.c_int, [vp, ctypes.c_char_p]),
+    'pdd_group_set_use_graph': (ctypes.c_int, [vp, ctypes.c_int]),
+    'pdd_group_set_tuning': (ctypes.c_int, [vp, i64, ctypes.c_char_p, i64]),
+    'pdd_group_graph_nodes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
 }
 
 EXPORTED_SYMBOLS = sorted(_SIGS)

@@ -6,6 +6,7 @@
 #include "host/plan.hpp"
 #include "host/pf_variance.hpp"
 #include "gpu/dedisperser.hpp"
+#include "gpu/dedisperser_group.hpp"
 #include "gpu/kernels.hpp"
 
 #include <algorithm>
@@ -17,7 +18,8 @@ using namespace pirate_amd;
 
 struct pdd_config { DedispersionConfig c; };
 struct pdd_plan { std::shared_ptr<DedispersionPlan> p; };
-struct pdd_dedisperser { std::unique_ptr<GpuDedisperser> d; };
+struct pdd_dedisperser { std::shared_ptr<GpuDedisperser> d; };
+struct pdd_group { std::unique_ptr<GpuDedisperserGroup> g; };
 
 static thread_local std::string g_last_error;
 
@@ -66,7 +68,7 @@ static int copy_string(const std::string &s, char *buf, size_t buflen, size_t *n
 extern "C" {
 
 const char *pdd_last_error(void) { return g_last_error.c_str(); }
-int pdd_abi_version(void) { return 3; }
+int pdd_abi_version(void) { return 4; }
 
 int pdd_config_from_yaml_file(const char *filename, pdd_config **out)
 {
@@ -255,7 +257,7 @@ int pdd_dedisperser_create(const pdd_plan *p, const pdd_dedisperser_params *prm,
             gp.initial_chunk = prm->initial_chunk;
             gp.enable_variances = prm->enable_variances != 0;
         }
-        *out = new pdd_dedisperser{std::make_unique<GpuDedisperser>(p->p, gp)};
+        *out = new pdd_dedisperser{std::make_shared<GpuDedisperser>(p->p, gp)};
     });
 }
 
@@ -307,6 +309,8 @@ int pdd_ipc_close(void *base, int device)
     PDD_TRY({
         PDD_CHECK(base, "null argument");
         hip_check(hipSetDevice(device), "hipSetDevice");
+        // work queued on this device may still read the mapping
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hip_check(hipIpcCloseMemHandle(base), "hipIpcCloseMemHandle");
     });
 }
@@ -481,6 +485,154 @@ int pdd_ringbuf_copy_launch(pdd_ringbuf_copy *k, void *ringbuf, int64_t ringbuf_
         launch_ringbuf_copy(ringbuf, k->d_oct, k->noct, k->seg_bytes, int(k->beams_per_batch), iframe, (hipStream_t) stream);
         hipError_t e = hipGetLastError();
         PDD_CHECK(e == hipSuccess, "pdd_ringbuf_copy_launch: launch failed: " << hipGetErrorString(e));
+    });
+}
+
+int pdd_dedisperser_spill_bytes(const pdd_dedisperser *dd, int64_t *nbytes)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d && nbytes, "null argument");
+        PDD_CHECK(dd->d->is_allocated, "pdd_dedisperser_spill_bytes: allocate() has not been called");
+        *nbytes = int64_t(dd->d->spill_bytes_per_batch());
+    });
+}
+
+int pdd_dedisperser_set_tuning(pdd_dedisperser *dd, const char *name, int64_t value)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d && name, "null argument");
+        dd->d->set_tuning(name, value);
+    });
+}
+
+int pdd_dedisperser_get_tuning(const pdd_dedisperser *dd, const char *name, int64_t *value)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d && name && value, "null argument");
+        *value = dd->d->get_tuning(name);
+    });
+}
+
+
+// ---- GpuDedisperserGroup ----
+
+#define PDD_TRY_G(gp, body)                                               \
+    try {                                                                 \
+        if (!(gp) || !(gp)->g) return fail("null group handle");          \
+        body;                                                             \
+        return 0;                                                         \
+    }                                                                     \
+    catch (std::exception &e) { (gp)->g->stop(e.what()); return fail(e.what()); } \
+    catch (...) { (gp)->g->stop("unknown C++ exception"); return fail("unknown C++ exception"); }
+
+int pdd_group_create(const pdd_plan *const *plans, int64_t nplans, const pdd_group_params *prm, pdd_group **out)
+{
+    PDD_TRY({
+        PDD_CHECK(plans && nplans > 0 && out, "null argument");
+        std::vector<std::shared_ptr<DedispersionPlan>> v;
+        for (int64_t i = 0; i < nplans; i++) {
+            PDD_CHECK(plans[i], "pdd_group_create: null plan " << i);
+            v.push_back(plans[i]->p);
+        }
+        GpuDedisperserGroupParams gp;
+        if (prm) {
+            gp.nslots = prm->nslots ? prm->nslots : 2;
+            gp.nbatches_out = prm->nbatches_out ? prm->nbatches_out : 2;
+            gp.device = prm->device;
+            gp.use_graph = prm->use_graph != 0;
+        }
+        *out = new pdd_group{std::make_unique<GpuDedisperserGroup>(v, gp)};
+    });
+}
+
+void pdd_group_destroy(pdd_group *g) { delete g; }
+
+int pdd_group_allocate(pdd_group *g) { PDD_TRY_G(g, g->g->allocate()); }
+
+int pdd_group_member(pdd_group *g, int64_t imember, pdd_dedisperser **dd)
+{
+    PDD_TRY({
+        PDD_CHECK(g && g->g && dd, "null argument");
+        PDD_CHECK(imember >= 0 && imember < int64_t(g->g->members.size()), "pdd_group_member: index out of range");
+        *dd = new pdd_dedisperser{g->g->members[imember]};
+    });
+}
+
+int pdd_group_gpu_bytes(const pdd_group *g, int64_t *nbytes)
+{
+    PDD_TRY({
+        PDD_CHECK(g && g->g && nbytes, "null argument");
+        *nbytes = int64_t(g->g->gpu_bytes());
+    });
+}
+
+int pdd_group_acquire_input(pdd_group *g, int64_t ichunk, void *stream, void **slots, int64_t nmembers)
+{
+    PDD_TRY_G(g, {
+        PDD_CHECK(slots && nmembers == int64_t(g->g->members.size()), "pdd_group_acquire_input: need one slot pointer per member");
+        std::vector<void *> v;
+        g->g->acquire_input(ichunk, (hipStream_t) stream, v);
+        for (int64_t m = 0; m < nmembers; m++)
+            slots[m] = v[m];
+    });
+}
+
+int pdd_group_release_input_and_launch(pdd_group *g, int64_t ichunk, void *stream)
+{
+    PDD_TRY_G(g, g->g->release_input_and_launch(ichunk, (hipStream_t) stream));
+}
+
+int pdd_group_acquire_output(pdd_group *g, int64_t ichunk, void *stream, int sync, void **out_max,
+                             void **out_argmax, int64_t n)
+{
+    PDD_TRY_G(g, {
+        int64_t tot = 0;
+        for (auto &m : g->g->members)
+            tot += m->ntrees;
+        PDD_CHECK(out_max && out_argmax && n == tot, "pdd_group_acquire_output: need " << tot
+                  << " (member, tree) pointers, got " << n);
+        g->g->acquire_output(ichunk, (hipStream_t) stream, sync != 0);
+        const long oslot = ichunk % g->g->nbatches_out;
+        int64_t k = 0;
+        for (auto &m : g->g->members)
+            for (long it = 0; it < m->ntrees; it++, k++) {
+                out_max[k] = m->output_slot(it, oslot, 0);
+                out_argmax[k] = m->output_slot(it, oslot, 1);
+            }
+    });
+}
+
+int pdd_group_release_output(pdd_group *g, int64_t ichunk, void *stream)
+{
+    PDD_TRY_G(g, g->g->release_output(ichunk, (hipStream_t) stream));
+}
+
+int pdd_group_set_use_graph(pdd_group *g, int use_graph)
+{
+    PDD_TRY_G(g, g->g->set_use_graph(use_graph != 0));
+}
+
+int pdd_group_set_tuning(pdd_group *g, int64_t imember, const char *name, int64_t value)
+{
+    PDD_TRY({
+        PDD_CHECK(g && g->g && name, "null argument");
+        g->g->set_tuning(imember, name, value);
+    });
+}
+
+int pdd_group_graph_nodes(const pdd_group *g, int64_t *n)
+{
+    PDD_TRY({
+        PDD_CHECK(g && g->g && n, "null argument");
+        *n = g->g->graph_nodes();
+    });
+}
+
+int pdd_group_stop(pdd_group *g, const char *reason)
+{
+    PDD_TRY({
+        PDD_CHECK(g && g->g, "null group handle");
+        g->g->stop(reason ? reason : "");
     });
 }
 

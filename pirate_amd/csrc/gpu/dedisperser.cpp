@@ -35,10 +35,14 @@ GpuDedisperser::GpuDedisperser(std::shared_ptr<DedispersionPlan> plan_, const Gp
     nstreams = plan->num_active_batches;
     ntrees = plan->ntrees;
 
+    // Defaults and lower bounds as the reference's (src_lib/GpuDedisperser.cpp:116-123).
     nbatches_out = params.nbatches_out ? params.nbatches_out : nstreams;
-    nbatches_wt = params.nbatches_wt ? params.nbatches_wt : nbatches_out;
+    nbatches_wt = params.nbatches_wt ? params.nbatches_wt : nstreams;
     num_consumers = params.num_consumers;
-    PDD_CHECK(nbatches_out > 0 && nbatches_wt > 0, "GpuDedisperser: nbatches_out and nbatches_wt must be > 0");
+    PDD_CHECK(nbatches_out >= nstreams, "GpuDedisperser: nbatches_out (" << nbatches_out
+              << ") must be >= num_active_batches (" << nstreams << ")");
+    PDD_CHECK(nbatches_wt >= nstreams, "GpuDedisperser: nbatches_wt (" << nbatches_wt
+              << ") must be >= num_active_batches (" << nstreams << ")");
     PDD_CHECK(num_consumers >= 1, "GpuDedisperser: num_consumers must be >= 1");
     PDD_CHECK(params.initial_chunk >= 0, "GpuDedisperser: initial_chunk must be >= 0");
 
@@ -53,22 +57,29 @@ GpuDedisperser::GpuDedisperser(std::shared_ptr<DedispersionPlan> plan_, const Gp
 GpuDedisperser::~GpuDedisperser()
 {
     // Copies of spilled ring rows may still be in flight on the copy streams.
-    for (hipStream_t x : {compute_stream, d2h_stream, h2d_stream})
+    for (hipStream_t x : {d2h_stream, h2d_stream})
         if (x)
             (void) hipStreamSynchronize(x);
-    for (hipStream_t x : branch_streams)
-        if (x && x != compute_stream)
-            (void) hipStreamSynchronize(x);
+    for (hipStream_t x : compute_streams)
+        (void) hipStreamSynchronize(x);
+    for (auto &bs : branch_streams)
+        for (size_t p = 1; p < bs.size(); p++)
+            if (bs[p])
+                (void) hipStreamSynchronize(bs[p]);
     auto freeit = [](DeviceArray &a) { if (a.ptr) (void) hipFree(a.ptr); a.ptr = nullptr; };
     freeit(input_slots);
-    freeit(grid_buf);
+    for (auto &a : grid_bufs)
+        freeit(a);
+    for (auto &v : lds_outs)
+        for (auto &a : v)
+            freeit(a);
     freeit(lds_state);
     freeit(grid_rows);
     freeit(grid_w);
     for (void *h : host_ring)
         if (h)
             (void) hipHostFree(h);
-    for (auto *v : {&lds_out, &rings, &ring_row_off, &ring_row_cap, &ring1_row_off, &ring1_row_cap, &spill_rows,
+    for (auto *v : {&rings, &ring_row_off, &ring_row_cap, &ring1_row_off, &ring1_row_cap, &spill_rows,
                     &stage2_lags, &sb, &d_main_emit, &d_stag,
                     &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_stag_goff, &d_out_stream, &s1_state, &s2_state, &weights, &out_var})
         for (auto &a : *v)
@@ -82,18 +93,21 @@ GpuDedisperser::~GpuDedisperser()
     for (hipStream_t x : {d2h_stream, h2d_stream})
         if (x)
             (void) hipStreamDestroy(x);
-    for (size_t p = 1; p < branch_streams.size(); p++)
-        if (branch_streams[p])
-            (void) hipStreamDestroy(branch_streams[p]);
-    for (hipEvent_t e : {ev_fork, ev_split_free})
-        if (e)
+    for (auto &bs : branch_streams)
+        for (size_t p = 1; p < bs.size(); p++)
+            if (bs[p])
+                (void) hipStreamDestroy(bs[p]);
+    if (ev_split_free)
+        (void) hipEventDestroy(ev_split_free);
+    for (auto &v : ev_join)
+        for (hipEvent_t e : v)
             (void) hipEventDestroy(e);
     for (auto *v : {&ev_input_free, &ev_input_ready, &ev_output_ready, &ev_output_released, &timing_event_pool,
-                    &ev_spill_s1, &ev_spill_d2h, &ev_spill_h2d, &ev_spill_scatter, &ev_join})
+                    &ev_spill_s1, &ev_spill_d2h, &ev_spill_h2d, &ev_spill_scatter, &ev_fork, &ev_batch_done})
         for (hipEvent_t e : *v)
             (void) hipEventDestroy(e);
-    if (compute_stream)
-        (void) hipStreamDestroy(compute_stream);
+    for (hipStream_t x : compute_streams)
+        (void) hipStreamDestroy(x);
 }
 
 
@@ -283,7 +297,10 @@ void GpuDedisperser::allocate()
     std::lock_guard<std::mutex> lk(mutex);
     PDD_CHECK(!is_allocated, "GpuDedisperser::allocate() called twice");
     HIP_CHECK(hipSetDevice(params.device));
-    HIP_CHECK(hipStreamCreateWithFlags(&compute_stream, hipStreamNonBlocking));
+    compute_streams.assign(nstreams, nullptr);
+    for (auto &x : compute_streams)
+        HIP_CHECK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    compute_stream = compute_streams[0];
 
     const long B = beams_per_batch, R = plan->config.toplevel_tree_rank, npri = plan->num_primary_trees;
     const long N = pow2(R);
@@ -302,7 +319,9 @@ void GpuDedisperser::allocate()
     };
 
     input_slots = dalloc(size_t(nstreams) * B * nfreq * nt_in * elt_bytes);
-    grid_buf = dalloc(size_t(B) * N * nt_in * elt_bytes);
+    grid_bufs.resize(nstreams);
+    for (auto &a : grid_bufs)
+        a = dalloc(size_t(B) * N * nt_in * elt_bytes);
 
     // Gridding CSR: per tree channel n, ascending input channels f in [if0, if1) with
     // overlap weights (reference: src_lib/TreeGriddingKernel.cu:77-100).
@@ -327,9 +346,10 @@ void GpuDedisperser::allocate()
         HIP_CHECK(hipMemcpy(grid_w.ptr, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
     }
 
-    lds_out.resize(npri);
-    for (long p = 1; p < npri; p++)
-        lds_out[p] = dalloc(size_t(B) * (N / 2) * (nt_in >> p) * elt_bytes);
+    lds_outs.assign(nstreams, std::vector<DeviceArray>(npri));
+    for (auto &v : lds_outs)
+        for (long p = 1; p < npri; p++)
+            v[p] = dalloc(size_t(B) * (N / 2) * (nt_in >> p) * elt_bytes);
     if (npri > 1) {
         long s = plan->stage1_dd_rank[1] + 1;
         lds_state_bstride = lds_state_nelts(int(R), int(npri), int(s));
@@ -518,11 +538,15 @@ void GpuDedisperser::allocate()
         HIP_CHECK(hipStreamCreateWithFlags(&d2h_stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&h2d_stream, hipStreamNonBlocking));
     }
-    branch_streams.assign(npri, nullptr);
-    for (long p = 1; p < npri; p++)
-        HIP_CHECK(hipStreamCreateWithFlags(&branch_streams[p], hipStreamNonBlocking));
-    mkev(ev_join, npri);
-    HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    branch_streams.assign(nstreams, std::vector<hipStream_t>(npri, nullptr));
+    ev_join.resize(nstreams);
+    for (long s = 0; s < nstreams; s++) {
+        for (long p = 1; p < npri; p++)
+            HIP_CHECK(hipStreamCreateWithFlags(&branch_streams[s][p], hipStreamNonBlocking));
+        mkev(ev_join[s], npri);
+    }
+    mkev(ev_fork, nstreams);
+    mkev(ev_batch_done, nbatches);
     HIP_CHECK(hipEventCreateWithFlags(&ev_split_free, hipEventDisableTiming));
     mkev(ev_input_free, nstreams);
     mkev(ev_input_ready, nstreams);
@@ -531,17 +555,84 @@ void GpuDedisperser::allocate()
     // Every slot starts out "free": record once so that first waits return immediately.
     for (auto &e : ev_input_free) HIP_CHECK(hipEventRecord(e, compute_stream));
     for (auto &e : ev_output_released) HIP_CHECK(hipEventRecord(e, compute_stream));
+    for (auto &e : ev_batch_done) HIP_CHECK(hipEventRecord(e, compute_stream));
     HIP_CHECK(hipStreamSynchronize(compute_stream));
 
     is_allocated = true;
 }
 
 
-void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t st)
+size_t GpuDedisperser::spill_bytes_per_batch() const
+{
+    size_t n = 0;
+    for (size_t p = 0; p < nspill.size(); p++)
+        n += size_t(beams_per_batch) * nspill[p] * (nt_in >> p) * elt_bytes;
+    return n;
+}
+
+
+void GpuDedisperser::set_tuning(const std::string &name, long value)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    if (name == "pf_target_blocks") {
+        PDD_CHECK(value >= 1 && value <= (1L << 20), "set_tuning(): pf_target_blocks must be in [1, 2^20]");
+        tuning.pf_target_blocks = value;
+    }
+    else if (name == "grid_xcd_run") {
+        PDD_CHECK(value >= 0 && value <= 64, "set_tuning(): grid_xcd_run must be in [0, 64]");
+        tuning.grid_xcd_run = int(value);
+    }
+    else if (name == "fork") {
+        PDD_CHECK(value >= -1 && value <= 2, "set_tuning(): fork must be -1, 0, 1 or 2");
+        tuning.fork = int(value);
+    }
+    else
+        throw pdd_error("set_tuning(): unknown launch parameter '" + name + "'");
+}
+
+long GpuDedisperser::get_tuning(const std::string &name) const
+{
+    if (name == "pf_target_blocks")
+        return tuning.pf_target_blocks;
+    if (name == "grid_xcd_run")
+        return tuning.grid_xcd_run;
+    if (name == "fork")
+        return tuning.fork;
+    throw pdd_error("get_tuning(): unknown launch parameter '" + name + "'");
+}
+
+
+void *GpuDedisperser::input_slot(long slot) const
+{
+    PDD_CHECK(is_allocated && slot >= 0 && slot < nstreams, "input_slot(): bad slot or not allocated");
+    return (char *) input_slots.ptr + size_t(slot) * beams_per_batch * nfreq * nt_in * elt_bytes;
+}
+
+void *GpuDedisperser::output_slot(long itree, long oslot, int which) const
+{
+    PDD_CHECK(is_allocated && itree >= 0 && itree < ntrees && oslot >= 0 && oslot < nbatches_out && (which == 0 || which == 1),
+              "output_slot(): bad tree, slot or array, or not allocated");
+    const DedispersionTree &t = plan->trees[itree];
+    const size_t n = size_t(beams_per_batch) * t.ndm_out * t.nt_out;
+    return which ? (void *) ((unsigned int *) out_argmax[itree].ptr + oslot * n)
+                 : (void *) ((char *) out_max[itree].ptr + oslot * n * elt_bytes);
+}
+
+
+void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t st,
+                                  long istream, const long *chunk_dev)
 {
     const long B = beams_per_batch, R = plan->config.toplevel_tree_rank, npri = plan->num_primary_trees;
     const long N = pow2(R);
     const int beam0 = int(ibatch * B);
+    PDD_CHECK(istream >= 0 && istream < nstreams, "launch_chain(): stream index out of range");
+    PDD_CHECK(!chunk_dev || std::all_of(nspill.begin(), nspill.end(), [](long n) { return n == 0; }),
+              "launch_chain(): a device chunk counter (graph replay) needs a plan without host spill");
+    const DeviceArray &grid_buf = grid_bufs[istream];
+    const std::vector<DeviceArray> &lds_out = lds_outs[istream];
+    const std::vector<hipStream_t> &bstreams = branch_streams[istream];
+    hipEvent_t ev_fork_s = ev_fork[istream];
+    last_stream = istream;
 
     {
         GridLdsArgs g;
@@ -564,9 +655,11 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         g.state = (float *) lds_state.ptr;
         g.state_bstride = lds_state_bstride;
         g.npri = int(npri);
+        g.xcd_run = tuning.grid_xcd_run;
         g.s = (npri > 1) ? int(plan->stage1_dd_rank[1] + 1) : 1;
         g.beam0 = beam0;
         g.chunk = ichunk;
+        g.chunk_dev = chunk_dev;
         hipEvent_t ev;
         timed_begin("grid_lds", st, ev);
         launch_grid_lds(dtype, g, st);
@@ -582,13 +675,15 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
     // 5.57 ms/step serial, 5.66 with the branches from the gridding on), while the
     // peak finders and the subband stage-2 kernels are bound by VALU or LDS latency and
     // leave bandwidth to share (chime_sb1-shaped, fp16: 15.68 -> 15.11 ms).
-    const bool late_fork = tables[0].fast;
-    if (!late_fork)
-        HIP_CHECK(hipEventRecord(ev_fork, st));
+    // tuning.fork == 2: every branch on 'st' in order (one kernel at a time, for per-kernel timing).
+    const bool serial = (tuning.fork == 2);
+    const bool late_fork = (tuning.fork < 0) ? tables[0].fast : (tuning.fork == 1);
+    if (!late_fork && !serial)
+        HIP_CHECK(hipEventRecord(ev_fork_s, st));
     for (long p = 0; p < npri; p++) {
-        hipStream_t sp = p ? branch_streams[p] : st;
-        if (p)
-            HIP_CHECK(hipStreamWaitEvent(sp, ev_fork, 0));
+        hipStream_t sp = (p && !serial) ? bstreams[p] : st;
+        if (sp != st)
+            HIP_CHECK(hipStreamWaitEvent(sp, ev_fork_s, 0));
         Stage1Args s;
         const long Tp = nt_in >> p;
         s.in = p ? lds_out[p].ptr : grid_buf.ptr;
@@ -605,6 +700,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
         s.Tp = int(Tp);
         s.beam0 = beam0;
         s.chunk = ichunk;
+        s.chunk_dev = chunk_dev;
         hipEvent_t ev;
         const std::string nm = "stage1_p" + std::to_string(p);
         timed_begin(nm, sp, ev);
@@ -651,6 +747,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             s2.beam0 = beam0;
             s2.amb1_dd1 = t.amb_rank;
             s2.chunk = ichunk;
+            s2.chunk_dev = chunk_dev;
             s2.out_stream = (const int *) d_out_stream[it].ptr;
             hipEvent_t ev2;
             const std::string nm2 = "stage2_t" + std::to_string(it);
@@ -666,8 +763,8 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             else
                 launch_stage2(dtype, s2, sp);
             timed_end(nm2, sp, ev2);
-            if (late_fork && it == 0)
-                HIP_CHECK(hipEventRecord(ev_fork, st));
+            if (late_fork && !serial && it == 0)
+                HIP_CHECK(hipEventRecord(ev_fork_s, st));
 
             PfArgs f;
             f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
@@ -696,6 +793,7 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             f.nt_wt = int(t.nt_wt);
             f.nt_out = int(t.nt_out);
             f.beam0 = beam0;
+            f.target_blocks = tuning.pf_target_blocks;
             if (params.enable_variances)
                 f.out_var = (double *) out_var[it].ptr + oslot * B * t.ndm_out * T.M * t.nprofiles;
             const std::string nmp = "pf_t" + std::to_string(it);
@@ -703,9 +801,9 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             launch_pf(dtype, f, sp);
             timed_end(nmp, sp, ev2);
         }
-        if (p) {
-            HIP_CHECK(hipEventRecord(ev_join[p], sp));
-            HIP_CHECK(hipStreamWaitEvent(st, ev_join[p], 0));
+        if (sp != st) {
+            HIP_CHECK(hipEventRecord(ev_join[istream][p], sp));
+            HIP_CHECK(hipStreamWaitEvent(st, ev_join[istream][p], 0));
         }
     }
     HIP_CHECK(hipGetLastError());
@@ -807,6 +905,7 @@ void *GpuDedisperser::acquire_input(long seq_id, hipStream_t stream)
     std::unique_lock<std::mutex> lk(mutex);
     check_stopped_locked();
     PDD_CHECK(is_allocated, "GpuDedisperser::acquire_input(): allocate() has not been called");
+    PDD_CHECK(!group_member, "GpuDedisperser::acquire_input(): this dedisperser is driven by a GpuDedisperserGroup");
     if (seq_id != curr_input_acquire || curr_input_acquire != curr_input_release) {
         std::ostringstream ss;
         ss << "GpuDedisperser::acquire_input(): expected seq_id=" << curr_input_release << ", got " << seq_id
@@ -848,18 +947,24 @@ void GpuDedisperser::release_input_and_launch(long seq_id, hipStream_t stream)
     });
     check_stopped_locked();
 
+    // Input slot s runs on compute stream s (reference: src_lib/GpuDedisperser.cpp:998); the
+    // batch's previous chunk, which may have run on another stream, must have finished
+    // (its tree states, lag ring and peak-finder rows carry over; :1010-1013).
     const long slot = seq_id % nstreams;
     const long ichunk = seq_id / nbatches, ibatch = seq_id % nbatches;
+    hipStream_t cs = compute_streams[slot];
     HIP_CHECK(hipEventRecord(ev_input_ready[slot], stream));
-    HIP_CHECK(hipStreamWaitEvent(compute_stream, ev_input_ready[slot], 0));
+    HIP_CHECK(hipStreamWaitEvent(cs, ev_input_ready[slot], 0));
+    HIP_CHECK(hipStreamWaitEvent(cs, ev_batch_done[ibatch], 0));
     for (long c = 0; c < num_consumers; c++)
-        HIP_CHECK(hipStreamWaitEvent(compute_stream, ev_output_released[oslot * num_consumers + c], 0));
+        HIP_CHECK(hipStreamWaitEvent(cs, ev_output_released[oslot * num_consumers + c], 0));
 
     const void *in = (char *) input_slots.ptr + size_t(slot) * beams_per_batch * nfreq * nt_in * elt_bytes;
-    launch_chain(in, ichunk, ibatch, oslot, seq_id % nbatches_wt, compute_stream);
+    launch_chain(in, ichunk, ibatch, oslot, seq_id % nbatches_wt, cs, slot);
 
-    HIP_CHECK(hipEventRecord(ev_input_free[slot], compute_stream));
-    HIP_CHECK(hipEventRecord(ev_output_ready[oslot], compute_stream));
+    HIP_CHECK(hipEventRecord(ev_batch_done[ibatch], cs));
+    HIP_CHECK(hipEventRecord(ev_input_free[slot], cs));
+    HIP_CHECK(hipEventRecord(ev_output_ready[oslot], cs));
     curr_input_release++;
     curr_launched = seq_id + 1;
     cv.notify_all();
@@ -873,6 +978,7 @@ void GpuDedisperser::acquire_output(long consumer, long seq_id, hipStream_t stre
     std::unique_lock<std::mutex> lk(mutex);
     check_stopped_locked();
     PDD_CHECK(consumer >= 0 && consumer < num_consumers, "acquire_output(): consumer out of range");
+    PDD_CHECK(!group_member, "GpuDedisperser::acquire_output(): this dedisperser is driven by a GpuDedisperserGroup");
     long &acq = curr_output_acquire[consumer];
     long &rel = curr_output_release[consumer];
     bool ok = (seq_id == acq) && (params.synchronous ? (acq == rel) : true);
@@ -1093,12 +1199,13 @@ size_t GpuDedisperser::debug_copy(const std::string &name, void *host, size_t ma
 {
     std::lock_guard<std::mutex> lk(mutex);
     PDD_CHECK(is_allocated, "debug_copy(): not allocated");
-    HIP_CHECK(hipStreamSynchronize(compute_stream));
+    for (hipStream_t x : compute_streams)
+        HIP_CHECK(hipStreamSynchronize(x));
     const DeviceArray *a = nullptr;
     if (name == "grid")
-        a = &grid_buf;
+        a = &grid_bufs[last_stream];
     else if (name.rfind("lds", 0) == 0)
-        a = &lds_out.at(std::stol(name.substr(3)));
+        a = &lds_outs[last_stream].at(std::stol(name.substr(3)));
     else if (name.rfind("sb", 0) == 0)
         a = &sb.at(std::stol(name.substr(2)));
     else if (name.rfind("ring", 0) == 0)

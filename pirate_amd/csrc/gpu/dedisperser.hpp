@@ -3,7 +3,8 @@
 // src_lib/GpuDedisperser.cpp:742-984 (acquire_input / release_input_and_launch_dd_kernels /
 // acquire_output / release_output cursors, back-pressure and stop semantics).
 //
-// One compute stream per dedisperser runs, per (chunk, beam batch):
+// One compute stream per active batch (num_active_batches of them, as the reference's
+// stream pool: include/pirate/CudaStreamPool.hpp:20-50) runs, per (chunk, beam batch):
 //   tree gridding -> lagged downsampling -> stage-1 trees -> lag ring ->
 //   stage-2 trees + subband streams -> peak finding -> output ring slot.
 // The caller's streams synchronize with it through events only.
@@ -24,12 +25,21 @@ namespace pirate_amd {
 
 struct GpuDedisperserParams {
     long nbatches_out = 0;     // output ring slots (0 = num_active_batches)
-    long nbatches_wt = 0;      // weight ring slots (0 = nbatches_out)
+    long nbatches_wt = 0;      // weight ring slots (0 = num_active_batches)
     long num_consumers = 1;
     int device = 0;
     bool synchronous = false;
     long initial_chunk = 0;
     bool enable_variances = false;   // peak finder also writes out_var (float64) per output slot
+};
+
+// Launch shapes that change no result (only speed), settable between chunks; the values
+// below are the defaults, the autotuner's table overrides them per workload.
+struct LaunchTuning {
+    long pf_target_blocks = 2048;   // peak finder: fewest time segments giving this many workgroups
+    int grid_xcd_run = 8;           // gridding: consecutive channel pairs per XCD (0: launch order)
+    int fork = -1;                  // primary trees p >= 1 start: -1 auto, 0 after the gridding,
+                                    // 1 after tree 0's stage 2, 2 serial (no branch streams)
 };
 
 struct DeviceArray {
@@ -46,6 +56,14 @@ public:
 
     std::shared_ptr<DedispersionPlan> plan;
     GpuDedisperserParams params;
+    LaunchTuning tuning;
+    // Set when a GpuDedisperserGroup drives this dedisperser: its own acquire/release entry
+    // points are then refused (the group launches its chains).
+    bool group_member = false;
+
+    // name in {"pf_target_blocks", "grid_xcd_run", "fork"}; throws on a bad name or value.
+    void set_tuning(const std::string &name, long value);
+    long get_tuning(const std::string &name) const;
 
     Dtype dtype;
     long nfreq, nt_in, total_beams, beams_per_batch, nbatches, nstreams, ntrees;
@@ -54,6 +72,8 @@ public:
 
     void allocate();
     bool is_allocated = false;
+    // PCIe bytes per beam batch of the host spill (each direction; 0 without max_gpu_clag).
+    size_t spill_bytes_per_batch() const;
     size_t gpu_bytes_allocated = 0;
     size_t host_bytes_allocated = 0;  // pinned host memory of spilled lag-ring rows
 
@@ -83,8 +103,17 @@ public:
 
     // Launches the chain for (ichunk, ibatch) on 'stream' without the acquire/release
     // protocol; the input is read from 'input' (device, (B, nfreq, nt_in), storage dtype)
-    // and the outputs go to output slot 'oslot'. Used by the timing harness.
-    void launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t stream);
+    // and the outputs go to output slot 'oslot'. 'istream' selects the set of per-stream
+    // intermediate buffers and branch streams (< num_active_batches). With 'chunk_dev' set,
+    // the kernels read the chunk index from that device address instead of 'ichunk' (for a
+    // captured HIP graph that is replayed for every chunk).
+    void launch_chain(const void *input, long ichunk, long ibatch, long oslot, long wslot, hipStream_t stream,
+                      long istream = 0, const long *chunk_dev = nullptr);
+
+    // Device address of input slot 'slot' (< num_active_batches) and of tree itree's output
+    // slot 'oslot' (which: 0 = out_max, 1 = out_argmax), for callers that drive launch_chain.
+    void *input_slot(long slot) const;
+    void *output_slot(long itree, long oslot, int which) const;
 
     // Output ring export to a consumer process: a 64-byte hipIpcMemHandle_t of the device
     // allocation that holds every tree's out_max and out_argmax ring (nbatches_out slots of
@@ -100,13 +129,19 @@ public:
     // launch (kernel names: gridding, lds, stage1_p<p>, stage2_t<i>, pf_t<i>). The report
     // lists "name launches total_ms" lines, accumulated since timing was enabled.
     void set_kernel_timing(bool enable);
+    bool kernel_timing_enabled() const { return timing; }
     std::string kernel_timing_report();
 
+    // Input slot s (seq_id % num_active_batches) is processed on compute_streams[s], with its
+    // own gridded and downsampled buffers, so that active batches run concurrently; the next
+    // chunk of a batch waits for the previous one (ev_batch_done) when it runs on another
+    // stream. compute_stream is compute_streams[0].
     hipStream_t compute_stream = nullptr;
-    // Primary trees p >= 1 run their stage 1, stage 2 and peak finding on branch_streams[p]
+    std::vector<hipStream_t> compute_streams;
+    // Primary trees p >= 1 run their stage 1, stage 2 and peak finding on branch_streams[s][p]
     // (branch 0 is the calling stream), forked after the gridding and joined before the chain
     // returns, so that the primary trees' kernels overlap.
-    std::vector<hipStream_t> branch_streams;
+    std::vector<std::vector<hipStream_t>> branch_streams;
 
     // Per-tree stage-2 tables (host copies, exposed for tests).
     struct TreeTables {
@@ -131,8 +166,9 @@ private:
     std::vector<long> curr_output_acquire, curr_output_release;
 
     DeviceArray input_slots;          // (nstreams, B, nfreq, nt_in)
-    DeviceArray grid_buf;             // (B, 2^R, nt_in)
-    std::vector<DeviceArray> lds_out; // [p] (B, 2^(R-1), nt_in >> p), p >= 1
+    std::vector<DeviceArray> grid_bufs;               // [s] (B, 2^R, nt_in)
+    std::vector<std::vector<DeviceArray>> lds_outs;   // [s][p] (B, 2^(R-1), nt_in >> p), p >= 1
+    long last_stream = 0;             // stream of the latest launch (debug_copy of "grid", "lds<p>")
     std::vector<DeviceArray> rings;   // [p]
     std::vector<DeviceArray> ring_row_off, ring_row_cap;   // [p] device (long), stage-2 (read) side
     std::vector<long> ring_bstride;
@@ -151,8 +187,10 @@ private:
     std::vector<long> nspill, outbox_off, inbox_off, host_bstride;   // [p]
     hipStream_t d2h_stream = nullptr, h2d_stream = nullptr;
     std::vector<hipEvent_t> ev_spill_s1, ev_spill_d2h, ev_spill_h2d, ev_spill_scatter;   // [p * nbatches + batch]
-    hipEvent_t ev_fork = nullptr, ev_split_free = nullptr;
-    std::vector<hipEvent_t> ev_join;                          // [p]
+    hipEvent_t ev_split_free = nullptr;
+    std::vector<hipEvent_t> ev_fork;                          // [s]
+    std::vector<std::vector<hipEvent_t>> ev_join;             // [s][p]
+    std::vector<hipEvent_t> ev_batch_done;                    // [batch]: latest chain of the batch
     std::vector<DeviceArray> spill_rows;                     // [p] device int: ring row of each outbox row
     std::vector<DeviceArray> ring1_row_off, ring1_row_cap;   // [p] stage-1 (write) side
     std::vector<void *> host_ring;                           // [p] pinned, (total_beams, host_bstride)

@@ -16,6 +16,14 @@ namespace pirate_amd {
 
 static constexpr float kRsqrt2 = 0.7071067811865476f;
 
+// Chunk index of a launch: read from device memory when the launch belongs to a replayed HIP
+// graph (whose last node advances the counter), else the launch argument.
+template <typename A>
+__device__ __forceinline__ long launch_chunk(const A &a)
+{
+    return a.chunk_dev ? *a.chunk_dev : a.chunk;
+}
+
 __device__ __forceinline__ float ld_st(const float *p) { return *p; }
 __device__ __forceinline__ float ld_st(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ float ld_st(const __hip_bfloat16 *p) { return __bfloat162float(*p); }

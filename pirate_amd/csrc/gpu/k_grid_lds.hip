@@ -27,8 +27,8 @@
 // float32 and round only at their output stores.
 //
 // Workgroup-to-pair mapping: workgroups are dispatched to the 8 XCDs in turn; each XCD takes
-// runs of 8 consecutive workgroup slots (8 channel pairs when one workgroup covers a pair's
-// whole chunk) and the 8 XCDs' runs are adjacent, so the device
+// runs of a.xcd_run (default 8) consecutive workgroup slots (8 channel pairs when one
+// workgroup covers a pair's whole chunk) and the 8 XCDs' runs are adjacent, so the device
 // streams one contiguous region of 64 pairs at a time while neighbouring pairs, which share
 // boundary input channels, share an XCD's L2. Measured on MI355X (chord16k): launch order
 // (pair = workgroup index) 1.27 ms, runs of 8 pairs 1.18 ms, and each XCD a contiguous
@@ -43,7 +43,6 @@
 namespace pirate_amd {
 
 static constexpr int kGlWaves = 4;
-static constexpr int kGlRun = 8;     // consecutive channel pairs per XCD run
 
 long lds_state_nelts(int R, int npri, int s)
 {
@@ -128,9 +127,10 @@ k_grid_lds(GridLdsArgs a)
     const int ntiles = (ngroups + 63) / 64;
     const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;          // workgroups per channel pair
     int blk = blockIdx.x;
-    if (gridDim.x % (8 * kGlRun) == 0) {
+    const int run = a.xcd_run;
+    if (run > 0 && gridDim.x % (8 * run) == 0) {
         const int x = blk & 7, slot = blk >> 3;                 // workgroup blk runs on XCD blk mod 8
-        blk = (slot / kGlRun) * (8 * kGlRun) + x * kGlRun + (slot % kGlRun);
+        blk = (slot / run) * (8 * run) + x * run + (slot % run);
     }
     const int j = blk / tpb;                                       // channel pair
     const int tile = (blk - j * tpb) * kGlWaves + (threadIdx.x >> 6);
@@ -179,8 +179,9 @@ k_grid_lds(GridLdsArgs a)
         const int lcap = lmask + 1;
         const long half = long(NPRI - 1) * nrows * (1 + lcap);
         float *st = a.state + long(a.beam0 + b) * a.state_bstride;
-        const float *rd = st + (a.chunk & 1) * half;            // written by the previous chunk
-        float *wr = st + ((a.chunk + 1) & 1) * half;            // read by the next chunk
+        const long chunk = launch_chunk(a);
+        const float *rd = st + (chunk & 1) * half;              // written by the previous chunk
+        float *wr = st + ((chunk + 1) & 1) * half;              // read by the next chunk
         const float *rd_small = rd, *rd_large = rd + long(NPRI - 1) * nrows;
         float *wr_small = wr, *wr_large = wr + long(NPRI - 1) * nrows;
 

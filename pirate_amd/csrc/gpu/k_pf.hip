@@ -410,11 +410,9 @@ static void launch_pf_lw(const PfArgs &a, hipStream_t st, dim3 grid)
     }
 }
 
-// Time segments per chunk: the fewest (a power of two) that give at least kPfTargetBlocks
-// workgroups, within segments of whole tiles of at least nwarm*TL + flmax positions (see
-// the rotation in k_pf). With out_var, one segment.
-static constexpr long kPfTargetBlocks = 2048;
-
+// Time segments per chunk: the fewest (a power of two) that give at least a.target_blocks
+// workgroups (2048 unless tuned), within segments of whole tiles of at least
+// nwarm*TL + flmax positions (see the rotation in k_pf). With out_var, one segment.
 static int pf_nseg(const PfArgs &a, long nblocks)
 {
     if (a.out_var)
@@ -422,7 +420,7 @@ static int pf_nseg(const PfArgs &a, long nblocks)
     const int TL = pf_tile_len(a.Dout);
     const int nwarm = (a.tpad + TL - 1) / TL;
     int nseg = 1;
-    while (nblocks * nseg < kPfTargetBlocks) {
+    while (nblocks * nseg < a.target_blocks) {
         const int n2 = 2 * nseg;
         if (a.Tp % (TL * n2) != 0 || a.Tp / n2 < nwarm * TL + a.flmax)
             break;

@@ -222,10 +222,11 @@ k_stage2(Stage2Args a)
         hist[i] = st[i];     // hist and shist are contiguous
     // ring row of tree row fc at this chunk's time 0: (offset + position, samples to the
     // wrap, capacity)
+    const long chunk = launch_chunk(a);
     for (int fc = tid; fc < TD::D; fc += kTreeThreads) {
         const long rr = (long(fc) << a.amb) + j;
         const long cap = a.ring.row_cap[rr];
-        const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+        const long pos = ring_pos(chunk * a.Tp - a.row_lag[rr], cap);
         desc[fc] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
     }
 

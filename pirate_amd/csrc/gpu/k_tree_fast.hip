@@ -265,16 +265,17 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     const long gbeam = long(a.beam0) + b;
 
     // ---- row descriptors
+    const long chunk = launch_chunk(a);
     for (int i = threadIdx.x; i < D; i += blockDim.x) {
         if constexpr (!STAGE2) {
             const long rr = long(tr) * D + i;
             const long cap = a.ring.row_cap[rr];
-            const long pos = (a.chunk * a.Tp) % cap;
+            const long pos = (chunk * a.Tp) % cap;
             odesc[i] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
         } else {
             const long rr = (long(i) << a.amb) + tr;
             const long cap = a.ring.row_cap[rr];
-            const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+            const long pos = ring_pos(chunk * a.Tp - a.row_lag[rr], cap);
             idesc[i] = make_int4(int(a.ring.row_off[rr] + pos), int(cap - pos), int(cap), 0);
             const int sidx = a.out_stream[i];
             const long dpf_base = long(bitrev_dev(tr, a.amb)) << (DD - a.pf_rank);
@@ -524,11 +525,12 @@ k_stage2_split_a_reg(Stage2Args a, float *X, long split_nst)
     const Tst *ring = reinterpret_cast<const Tst *>(a.ring.base) + gbeam * a.ring.bstride;
     long rbase[16];
     int rwrap[16], rcap[16];
+    const long chunk = launch_chunk(a);
 #pragma unroll
     for (int l = 0; l < 16; l++) {
         const long rr = (long(h * 16 + l) << a.amb) + tr;
         const long cap = a.ring.row_cap[rr];
-        const long pos = ring_pos(a.chunk * a.Tp - a.row_lag[rr], cap);
+        const long pos = ring_pos(chunk * a.Tp - a.row_lag[rr], cap);
         rbase[l] = a.ring.row_off[rr] + pos;
         rwrap[l] = int(cap - pos);
         rcap[l] = int(cap);

@@ -29,8 +29,10 @@ struct GridLdsArgs {
     float *state;                        // per global beam, see lds_state_nelts()
     long state_bstride;
     int npri, s;                         // s = (stage-1 dd_rank of p>=1) + 1
+    int xcd_run = 8;                     // consecutive channel pairs per XCD (0: plain order)
     int beam0;                           // global index of the batch's first beam
     long chunk;                          // chunk index (selects the state parity)
+    const long *chunk_dev = nullptr;     // if set, the chunk index is read here (graph replay)
 };
 long lds_state_nelts(int R, int npri, int s);   // per beam
 void launch_grid_lds(Dtype dt, const GridLdsArgs &a, hipStream_t st);
@@ -52,6 +54,7 @@ struct Stage1Args {
     long state_bstride;       // per beam
     int B, amb, dd, Tp, beam0;
     long chunk;               // chunk index (ring time = chunk*Tp + t)
+    const long *chunk_dev = nullptr;   // if set, the chunk index is read here (graph replay)
 };
 
 
@@ -75,6 +78,7 @@ struct Stage2Args {
     long state_bstride;
     int B, amb, dd, Tp, beam0, amb1_dd1;   // amb1_dd1: stage-1 dd rank (ring row = fc*2^amb1_dd1 + j)
     long chunk;
+    const long *chunk_dev = nullptr;   // if set, the chunk index is read here (graph replay)
     const int *out_stream;    // register-tree path: (2^dd) final row -> stream, or -1
     const int *stag_goff;     // rank-8 split: staggered history offset within its row group
 };
@@ -125,6 +129,7 @@ struct PfArgs {
     int B, ndm_out, M, N, P, nlev, Tp, Dout, Dcore, ndm_wt, nt_wt, nt_out;
     int beam0;                // global index of the batch's first beam (sb rows are per global beam)
     int nseg = 1;             // time segments per chunk (set by launch_pf)
+    long target_blocks = 2048;   // launch_pf picks the fewest segments giving this many workgroups
     double *out_var = nullptr;   // optional (B, ndm_out, M, P) float64: per-chunk mean square of each
                                  // (multiplet, profile)'s weighted candidates
 };

@@ -15,9 +15,13 @@ _TYPESTR = {'float32': '<f4', 'float16': '<f2', 'bfloat16': '<V2', 'float64': '<
 
 
 class _DeviceView:
-    """Minimal __cuda_array_interface__ provider for torch.as_tensor()."""
+    """Minimal __cuda_array_interface__ provider for torch.as_tensor(). torch keeps this
+    object alive as long as any tensor viewing its memory, and this object keeps 'owner'
+    (the object whose device allocation the view points into) alive, so the allocation is
+    not freed under a live tensor."""
 
-    def __init__(self, ptr, shape, typestr):
+    def __init__(self, ptr, shape, typestr, owner=None):
+        self.owner = owner
         self.__cuda_array_interface__ = {'shape': tuple(int(s) for s in shape), 'typestr': typestr,
                                          'data': (int(ptr), False), 'version': 2, 'strides': None}
 
@@ -27,16 +31,17 @@ def _torch():
     return torch
 
 
-def device_view(ptr, shape, dtype):
-    """torch tensor viewing 'ptr' (no copy). dtype in {'float32','float16','bfloat16','uint32'}."""
+def device_view(ptr, shape, dtype, owner=None):
+    """torch tensor viewing 'ptr' (no copy). dtype in {'float32','float16','bfloat16','uint32'}.
+    The tensor holds a reference to 'owner' (the allocation's owner) while it lives."""
     torch = _torch()
     if dtype == 'uint32':
-        t = torch.as_tensor(_DeviceView(ptr, shape, '<i4'), device='cuda')
+        t = torch.as_tensor(_DeviceView(ptr, shape, '<i4', owner), device='cuda')
         return t   # int32 view of the uint32 tokens (same bits)
     if dtype == 'bfloat16':
-        t = torch.as_tensor(_DeviceView(ptr, shape, '<i2'), device='cuda')
+        t = torch.as_tensor(_DeviceView(ptr, shape, '<i2', owner), device='cuda')
         return t.view(torch.bfloat16)
-    return torch.as_tensor(_DeviceView(ptr, shape, _TYPESTR[dtype]), device='cuda')
+    return torch.as_tensor(_DeviceView(ptr, shape, _TYPESTR[dtype], owner), device='cuda')
 
 
 def _stream_ptr(stream):
@@ -92,6 +97,21 @@ class GpuDedisperser:
 
     def __init__(self, plan, stream_pool=None, cuda_device_id=0, num_consumers=1, nbatches_out=0, nbatches_wt=0,
                  initial_chunk=0, synchronous=False, enable_variances=False):
+        self._setup(plan, cuda_device_id, num_consumers, nbatches_out, nbatches_wt, initial_chunk, enable_variances)
+        p = L.DedisperserParams(nbatches_out, nbatches_wt, self.num_consumers, cuda_device_id, int(synchronous),
+                                initial_chunk, int(self.enable_variances))
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().pdd_dedisperser_create(plan._h, ctypes.byref(p), ctypes.byref(self._h)))
+
+    @classmethod
+    def _wrap(cls, plan, handle, cuda_device_id, nbatches_out):
+        """A GpuDedisperser over an existing C handle (a member of a GpuDedisperserGroup)."""
+        self = cls.__new__(cls)
+        self._setup(plan, cuda_device_id, 1, nbatches_out, 0, 0, False)
+        self._h = handle
+        return self
+
+    def _setup(self, plan, cuda_device_id, num_consumers, nbatches_out, nbatches_wt, initial_chunk, enable_variances):
         self.plan = plan
         self.config = plan.config
         self.dtype = plan.dtype
@@ -104,15 +124,15 @@ class GpuDedisperser:
         self.ntrees = plan.ntrees
         self.trees = plan.trees
         self.num_consumers = num_consumers if num_consumers > 0 else 1
+        # defaults and lower bounds as the reference's (src_lib/GpuDedisperser.cpp:116-123)
         self.nbatches_out = nbatches_out if nbatches_out > 0 else self.nstreams
-        self.nbatches_wt = nbatches_wt if nbatches_wt > 0 else self.nbatches_out
+        self.nbatches_wt = nbatches_wt if nbatches_wt > 0 else self.nstreams
+        if self.nbatches_out < self.nstreams or self.nbatches_wt < self.nstreams:
+            raise ValueError(f'GpuDedisperser: nbatches_out ({self.nbatches_out}) and nbatches_wt ({self.nbatches_wt}) '
+                             f'must be >= num_active_batches ({self.nstreams})')
         self.initial_chunk = initial_chunk
         self.cuda_device_id = cuda_device_id
         self.enable_variances = bool(enable_variances)
-        p = L.DedisperserParams(nbatches_out, nbatches_wt, self.num_consumers, cuda_device_id, int(synchronous),
-                                initial_chunk, int(self.enable_variances))
-        self._h = ctypes.c_void_p()
-        L.check(L.lib().pdd_dedisperser_create(plan._h, ctypes.byref(p), ctypes.byref(self._h)))
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -160,12 +180,31 @@ class GpuDedisperser:
         L.check(L.lib().pdd_dedisperser_gpu_bytes(self._h, ctypes.byref(n)))
         return n.value
 
+    def resource_tracker(self):
+        """Per-kernel HBM bytes per beam batch, footprints and host-spill PCIe bytes (the
+        reference's GpuDedisperser::resource_tracker, include/pirate/ResourceTracker.hpp:14-75).
+        Needs allocate()."""
+        from .resources import ResourceTracker
+        n = ctypes.c_int64()
+        L.check(L.lib().pdd_dedisperser_spill_bytes(self._h, ctypes.byref(n)))
+        return ResourceTracker(self.plan, gmem_footprint=self.gpu_bytes(), hmem_footprint=self.host_bytes(),
+                               memcpy_h2g=n.value, memcpy_g2h=n.value)
+
+    def time(self, niterations, log=print):
+        """Timing loop of the reference's GpuDedisperser::time (src_lib/GpuDedisperser.cpp:1573-1690):
+        every iteration uploads an int4 chunk and its float16 scales/offsets from pinned host
+        memory on a copy stream, dequantizes it into the acquired input slot on the slot's
+        stream and runs the chain. Returns the steady-state figures (see timing.time_dedisperser)."""
+        from .timing import time_dedisperser
+        return time_dedisperser(self, niterations, log=log)
+
     # ---- low-level interface (reference: _acquire_input etc.) ----
 
     def _acquire_input(self, seq_id, stream_ptr):
         slot = ctypes.c_void_p()
         L.check(L.lib().pdd_acquire_input(self._h, seq_id, ctypes.c_void_p(stream_ptr), ctypes.byref(slot)))
-        return device_view(slot.value, (self.beams_per_batch, self.nfreq, self.nt_in), self.dtype)
+        self._next_seq = seq_id + 1
+        return device_view(slot.value, (self.beams_per_batch, self.nfreq, self.nt_in), self.dtype, owner=self)
 
     def _release_input_and_launch_dd_kernels(self, seq_id, stream_ptr):
         L.check(L.lib().pdd_release_input_and_launch(self._h, seq_id, ctypes.c_void_p(stream_ptr)))
@@ -179,13 +218,13 @@ class GpuDedisperser:
         mx, am = [], []
         for i, t in enumerate(self.trees):
             shape = (self.beams_per_batch, t.ndm_out, t.nt_out)
-            mx.append(device_view(om[i], shape, self.dtype))
-            am.append(device_view(oa[i], shape, 'uint32'))
+            mx.append(device_view(om[i], shape, self.dtype, owner=self))
+            am.append(device_view(oa[i], shape, 'uint32', owner=self))
         ov = None
         if self.enable_variances:
             vp = (ctypes.c_void_p * n)()
             L.check(L.lib().pdd_output_variances(self._h, seq_id, vp, n))
-            ov = [device_view(vp[i], (self.beams_per_batch, t.ndm_out, t.nmultiplets_M, t.nprofiles), 'float64')
+            ov = [device_view(vp[i], (self.beams_per_batch, t.ndm_out, t.nmultiplets_M, t.nprofiles), 'float64', owner=self)
                   for i, t in enumerate(self.trees)]
         ichunk = seq_id // self.nbatches
         return GpuDedisperserOutputs(mx, am, ichunk, ichunk + self.initial_chunk,
@@ -275,6 +314,16 @@ class GpuDedisperser:
     def stop(self, reason=''):
         L.check(L.lib().pdd_stop(self._h, reason.encode()))
 
+    def set_tuning(self, name, value):
+        """Sets a launch shape that changes no output, only speed: 'pf_target_blocks',
+        'grid_xcd_run' or 'fork' (see include/pirate_amd.h); applies from the next launch."""
+        L.check(L.lib().pdd_dedisperser_set_tuning(self._h, name.encode(), int(value)))
+
+    def get_tuning(self, name):
+        v = ctypes.c_int64()
+        L.check(L.lib().pdd_dedisperser_get_tuning(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
     def set_kernel_timing(self, enable):
         L.check(L.lib().pdd_kernel_timing(self._h, int(enable)))
 
@@ -301,7 +350,10 @@ class OutputRingView:
     GpuDedisperser.output_ipc_handle() in another process of the node (the reference's
     grouper side, src_lib/FrbGrouper.cpp:565-599). out_max(itree, slot) and
     out_argmax(itree, slot) are (beams_per_batch, ndm_out, nt_out) tensors on the GPU
-    (argmax tokens as int32 with the uint32 bits), valid until close()."""
+    (argmax tokens as int32 with the uint32 bits). Each returned tensor keeps this view
+    alive, so the mapping outlives every tensor unless close() is called explicitly; close()
+    waits for the device's queued work and then unmaps, after which the tensors must not be
+    used."""
 
     def __init__(self, handle, layout):
         self.layout = layout
@@ -320,7 +372,7 @@ class OutputRingView:
         dtype = self.layout['dtype'] if which == 0 else 'uint32'
         esize = 2 if dtype in ('float16', 'bfloat16') else 4
         off = t['max_offset' if which == 0 else 'argmax_offset'] + slot * B * ndm * nt * esize
-        return device_view(self._base.value + off, (B, ndm, nt), dtype)
+        return device_view(self._base.value + off, (B, ndm, nt), dtype, owner=self)
 
     def out_max(self, itree, slot):
         return self._view(itree, slot, 0)

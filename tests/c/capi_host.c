@@ -40,7 +40,7 @@ int main(void)
     double *fv, *tv;
     long i, n;
 
-    CHECK(pdd_abi_version() == 3, "abi version");
+    CHECK(pdd_abi_version() == 4, "abi version");
     CHECK(pdd_config_from_yaml_string(yaml, &cfg) == 0, "config from yaml");
     CHECK(pdd_plan_create(cfg, 1, &plan) == 0, "plan create");
     CHECK(pdd_plan_ntrees(plan, &ntrees) == 0 && ntrees == 1, "one tree");

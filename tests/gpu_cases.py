@@ -93,3 +93,21 @@ def cfg2_with_beams(nbeams, batch=None):
     d['beams_per_gpu'] = nbeams
     d['beams_per_batch'] = batch or nbeams
     return d
+
+
+# Batched heterogeneous beams: three beam groups of 32 beams at tree ranks 12/13/14
+# (4096/8192/16384 channels over 300-1500 MHz) with 3/4/5 primary trees (downsampling levels),
+# float16 storage, chunks of 2048 samples, run as one GpuDedisperserGroup (one graph launch per
+# chunk). Not fixed by the workload and chosen here: no subband search ([1]), max_width 16 ms
+# at every downsampling level, weight downsampling 64x64.
+def cfg4_group(rank, nbeams, dtype='float16'):
+    nfreq = {12: 4096, 13: 8192, 14: 16384}[rank]
+    npri = {12: 3, 13: 4, 14: 5}[rank]
+    widths = [16, 8, 4, 2, 1][:npri]
+    return dict(BASE, dtype=dtype, zone_nfreq=[nfreq], zone_freq_edges=[300, 1500], toplevel_tree_rank=rank,
+                time_samples_per_chunk=2048, beams_per_gpu=nbeams, beams_per_batch=nbeams, num_active_batches=1,
+                frequency_subband_counts=[1], primary_trees=[pt(0, w, 64, 64) for w in widths])
+
+
+def cfg4_groups(nbeams=32, dtype='float16'):
+    return [cfg4_group(r, nbeams, dtype) for r in (12, 13, 14)]

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     for name in sorted(declared):
         assert hasattr(L, name), f'{name} declared in pirate_amd.h but not exported'
     assert declared == set(_lib.EXPORTED_SYMBOLS)
-    assert L.pdd_abi_version() == 3
+    assert L.pdd_abi_version() == 4
 
 
 def test_plain_c_caller(tmp_path):
@@ -244,3 +244,43 @@ def test_bare_profile_variances_match_oracle(orc):
     from pirate_amd.dedisperser import bare_profile_variances
     for P in (1, 4, 7, 10, 13, 16):
         assert np.array_equal(bare_profile_variances(P), orc.bare_profile_variances(P))
+
+
+def test_nbatches_defaults_and_lower_bounds():
+    # Reference defaults: nbatches_out and nbatches_wt both default to num_active_batches and
+    # may not be below it (src_lib/GpuDedisperser.cpp:116-123); construction touches no device.
+    d = dict(golden()['configs']['toy'])
+    d.update(beams_per_gpu=4, beams_per_batch=2, num_active_batches=1)
+    plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
+    dd = pa.GpuDedisperser(plan, nbatches_out=2)
+    assert dd.nbatches_out == 2 and dd.nbatches_wt == 1
+    assert dd.wt_shape(0)[0] == 1
+    d.update(num_active_batches=2)
+    plan2 = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
+    with pytest.raises(ValueError, match='num_active_batches'):
+        pa.GpuDedisperser(plan2, nbatches_out=1)
+    with pytest.raises(ValueError, match='num_active_batches'):
+        pa.GpuDedisperser(plan2, nbatches_wt=1)
+
+
+def test_device_view_keeps_its_owner_alive():
+    # A tensor viewing dedisperser memory holds the _DeviceView (torch keeps the object that
+    # provides __cuda_array_interface__), and the view holds its owner, so the owner (whose
+    # destructor frees the memory) outlives every view.
+    import gc
+    import weakref
+    from pirate_amd.dedisperser import _DeviceView
+
+    class Owner:
+        pass
+
+    o = Owner()
+    ref = weakref.ref(o)
+    v = _DeviceView(0x1000, (2, 3), '<f4', owner=o)
+    del o
+    gc.collect()
+    assert ref() is not None
+    assert v.__cuda_array_interface__['shape'] == (2, 3)
+    del v
+    gc.collect()
+    assert ref() is None

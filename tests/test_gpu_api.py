@@ -155,3 +155,23 @@ def test_fill_analytic_weights_bad_variances_stop():
         dd.fill_analytic_weights(np.zeros(dd.nfreq))
     with pytest.raises(RuntimeError, match='stopped'):
         dd._acquire_input(0, 0)
+
+
+def test_views_keep_the_dedisperser_alive():
+    # Input and output tensors hold the dedisperser, so dropping the last Python reference to
+    # it while a view lives does not free the memory under the view.
+    import gc
+    import weakref
+    dd = make()
+    ref = weakref.ref(dd)
+    push(dd, 0)
+    out = dd._acquire_output(0, 0, 0, sync=True)
+    om = out.out_max[0]
+    dd._release_output(0, 0, 0)
+    del dd, out
+    gc.collect()
+    assert ref() is not None
+    om.sum().item()       # the memory is still mapped and owned
+    del om
+    gc.collect()
+    assert ref() is None
