@@ -1,5 +1,8 @@
 """Runs the GPU dedisperser and the CPU oracle on the same seeded inputs and weights."""
 
+import queue
+import threading
+
 import numpy as np
 
 from oracle import oracle as orc
@@ -138,3 +141,38 @@ def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print):
         log(f'{dtype} tree {it}: max rel err {stats[it]["max"]:.2e}, rms {stats[it]["rms"]:.2e}, '
             f'argmax candidate max rel err {stats[it]["tok_max"]:.2e} over {e.size} (dm, t) outputs')
     return stats
+
+
+class OracleWorker(threading.Thread):
+    """One oracle instance per (member, beam), fed (ichunk, x, gpu_max, gpu_argmax) items; the
+    oracle's C calls release the GIL, so workers run in parallel."""
+
+    def __init__(self, cfg_dict, weights, storage, tag):
+        super().__init__(daemon=True)
+        from oracle import oracle as orc
+        d1 = dict(cfg_dict, beams_per_gpu=1, beams_per_batch=1)
+        self.ref = orc.Dedisperser(oracle_config(d1), 1, storage=storage)
+        for it, w in enumerate(weights):
+            self.ref.set_weights(it, w)
+        self.q = queue.Queue(maxsize=4)
+        self.bad = []
+        self.nchecked = 0
+        self.tag = tag
+        self.err = None
+
+    def run(self):
+        try:
+            while True:
+                item = self.q.get()
+                if item is None:
+                    return
+                ichunk, x, gmax, garg = item
+                self.ref.dedisperse(x, ichunk, 0)
+                for it in range(len(gmax)):
+                    rm, ra = self.ref.output(it)
+                    nm, na = int(np.sum(gmax[it] != rm)), int(np.sum(garg[it] != ra))
+                    if nm or na:
+                        self.bad.append(dict(tag=self.tag, ichunk=ichunk, itree=it, n_max=nm, n_argmax=na))
+                self.nchecked += 1
+        except Exception as e:       # noqa: BLE001 -- reported by the test
+            self.err = e

@@ -14,15 +14,13 @@ Checks, bit for bit:
     lags (10 and 34 chunks) and the rank-14 group's primary trees 0 and 1 (18 chunks).
 """
 
-import queue
 import sys
-import threading
 
 import numpy as np
 import pytest
 
 from gpu_cases import BASE, cfg4_groups, pt
-from gpu_harness import oracle_config, random_weights
+from gpu_harness import OracleWorker, random_weights
 
 pytestmark = pytest.mark.gpu
 
@@ -53,41 +51,6 @@ def max_lag_chunks(plan, trees=None):
         lag = (2**t.dd_rank - 1) * (2**t.amb_rank - 1 + (2**t.amb_rank if p > 0 else 0))
         nmax = max(nmax, -(-lag // t.nt_ds) + 2)
     return nmax
-
-
-class OracleWorker(threading.Thread):
-    """One oracle instance per (member, beam), fed (ichunk, x, gpu_max, gpu_argmax) items; the
-    oracle's C calls release the GIL, so workers run in parallel."""
-
-    def __init__(self, cfg_dict, weights, storage, tag):
-        super().__init__(daemon=True)
-        from oracle import oracle as orc
-        d1 = dict(cfg_dict, beams_per_gpu=1, beams_per_batch=1)
-        self.ref = orc.Dedisperser(oracle_config(d1), 1, storage=storage)
-        for it, w in enumerate(weights):
-            self.ref.set_weights(it, w)
-        self.q = queue.Queue(maxsize=4)
-        self.bad = []
-        self.nchecked = 0
-        self.tag = tag
-        self.err = None
-
-    def run(self):
-        try:
-            while True:
-                item = self.q.get()
-                if item is None:
-                    return
-                ichunk, x, gmax, garg = item
-                self.ref.dedisperse(x, ichunk, 0)
-                for it in range(len(gmax)):
-                    rm, ra = self.ref.output(it)
-                    nm, na = int(np.sum(gmax[it] != rm)), int(np.sum(garg[it] != ra))
-                    if nm or na:
-                        self.bad.append(dict(tag=self.tag, ichunk=ichunk, itree=it, n_max=nm, n_argmax=na))
-                self.nchecked += 1
-        except Exception as e:       # noqa: BLE001 -- reported by the test
-            self.err = e
 
 
 def host_outputs(out, b):
