@@ -253,17 +253,17 @@ int pdd_dedisperser_get_tuning(const pdd_dedisperser *dd, const char *name, int6
  * primary trees, downsampling levels), all its beams in one batch (beams_per_batch ==
  * beams_per_gpu), every plan with the same chunk (time_samples_per_chunk, time_sample_ms),
  * none spilling to the host (no max_gpu_clag). The group owns nslots input slots per member
- * (the caller fills chunk c+1 while chunk c runs). Per chunk,
- * every member's chain runs as independent branches of one HIP graph, captured once per
- * (input slot, output slot, weight slot) phase and replayed; the kernels read the chunk index
- * from a device counter that each replay advances. use_graph = 0 issues the same launches
- * one by one. */
+ * (the caller fills chunk c+1 while chunk c runs). Per chunk, every member's chain runs on a
+ * branch stream of its own, forked from and joined into one launch stream, so that the
+ * members' kernels run at once. use_graph must be 0: HIP graph replay of the chunk is refused
+ * in this build (its capture path ended in a segmentation fault on MI355X, cause not yet
+ * identified); pdd_group_create and pdd_group_set_use_graph fail for use_graph != 0. */
 typedef struct pdd_group pdd_group;
 typedef struct {
     int64_t nslots;          /* input slots per member (0 = 2) */
     int64_t nbatches_out;    /* output slots per member (0 = 2) */
     int32_t device;
-    int32_t use_graph;
+    int32_t use_graph;       /* must be 0 (see above) */
 } pdd_group_params;
 
 int pdd_group_create(const pdd_plan *const *plans, int64_t nplans, const pdd_group_params *params, pdd_group **out);
@@ -308,6 +308,52 @@ int pdd_kernel_timing_report(pdd_dedisperser *dd, char *buf, size_t buflen, size
 /* Test hook (no reference counterpart): copies an internal device buffer to the host.
  * name: "grid", "lds<p>", "sb<itree>", "ring<p>". With host == NULL only *nbytes is set. */
 int pdd_debug_copy(pdd_dedisperser *dd, const char *name, void *host, size_t maxbytes, size_t *nbytes);
+
+/* ---- learned peak scorer ----
+ *
+ * No counterpart in the reference: a small bf16 1-D convolutional network over (DM, time) tiles
+ * of the dedispersed output (64 rows x 256 samples), run on MFMA, that stands beside the boxcar
+ * statistic of the peak finder (PeakFindingKernel, src_lib/PeakFindingKernel.cu:252-456), which
+ * stays the fallback and the judge of its completeness. Network, parameter order and rounding:
+ * pirate_amd/csrc/gpu/scorer.hpp. All pointers are device pointers; all calls are asynchronous
+ * on 'stream' (a hipStream_t, NULL = default stream). */
+
+/* Tiles: element (beam, row, t) at base + beam*beam_stride + row*row_stride + t (elements of
+ * dtype 0 = float32, 1 = float16, 2 = bfloat16), valid for beam < nbeams, row < nrows,
+ * t_lo <= t < t_hi. origins: (ntiles, 3) int32 (beam, row0, t0), or NULL for tile i = (i, 0, 0).
+ * A tile outside the valid range gets a NaN logit and no gradient. */
+typedef struct {
+    const void *base;
+    int32_t dtype;
+    int64_t nbeams, beam_stride, nrows, row_stride, t_lo, t_hi;
+    const int32_t *origins;
+} pdd_tile_source;
+
+/* Number of float32 parameters (13905). */
+int64_t pdd_scorer_nparams(void);
+
+/* logits[i] of tile i (ntiles floats). */
+int pdd_scorer_forward(const float *params, const pdd_tile_source *src, int64_t ntiles, float *logits, void *stream);
+
+/* Gradient of the mean binary cross-entropy over the ntiles tiles with labels[i] in {0, 1}:
+ * grad (nparams floats), plus logits[i] and loss_terms[i] = softplus(logit) - label*logit. The
+ * per-work-group partial gradients live in a workspace that this call allocates and frees in
+ * stream order (hipMallocAsync / hipFreeAsync); their sum is formed in a fixed order, so the
+ * result does not depend on timing. */
+int pdd_scorer_grad(const float *params, const pdd_tile_source *src, const float *labels, int64_t ntiles,
+                    float *grad, float *logits, float *loss_terms, void *stream);
+
+/* One Adam step (step counted from 1) on params, with moment estimates m and v (nparams floats
+ * each, zero before step 1). */
+int pdd_scorer_adam(float *params, float *m, float *v, const float *grad, int64_t step, float lr, float beta1,
+                    float beta2, float eps, void *stream);
+
+/* Tile source over the peak-finder input rows of tree itree (its stage-2 output): rows
+ * idm_out * M + m of every global beam, t = chunk sample (t_lo = -history samples, t_hi = chunk
+ * samples of the tree). With a single full-band subband (M = 2^pf_rank multiplets), row r is
+ * DM trial r of the tree. The rows hold the latest chunk whose stage 2 has run on the beam's
+ * batch: read them after acquire_output of that chunk and before the batch's next launch. */
+int pdd_dedisperser_tree_rows(pdd_dedisperser *dd, int64_t itree, pdd_tile_source *src);
 
 #ifdef __cplusplus
 }

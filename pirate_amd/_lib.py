@@ -36,6 +36,11 @@ class DedisperserParams(ctypes.Structure):
                 ('enable_variances', ctypes.c_int32)]
 
 
+class TileSource(ctypes.Structure):
+    _fields_ = [('base', vp), ('dtype', ctypes.c_int32), ('nbeams', i64), ('beam_stride', i64), ('nrows', i64),
+                ('row_stride', i64), ('t_lo', i64), ('t_hi', i64), ('origins', vp)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     'pdd_last_error': (ctypes.c_char_p, []),
@@ -98,6 +103,12 @@ _SIGS = {
     'pdd_group_set_use_graph': (ctypes.c_int, [vp, ctypes.c_int]),
     'pdd_group_set_tuning': (ctypes.c_int, [vp, i64, ctypes.c_char_p, i64]),
     'pdd_group_graph_nodes': (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+    'pdd_scorer_nparams': (i64, []),
+    'pdd_scorer_forward': (ctypes.c_int, [vp, ctypes.POINTER(TileSource), i64, vp, vp]),
+    'pdd_scorer_grad': (ctypes.c_int, [vp, ctypes.POINTER(TileSource), vp, i64, vp, vp, vp, vp]),
+    'pdd_scorer_adam': (ctypes.c_int, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, vp]),
+    'pdd_dedisperser_tree_rows': (ctypes.c_int, [vp, i64, ctypes.POINTER(TileSource)]),
 }
 
 EXPORTED_SYMBOLS = sorted(_SIGS)

@@ -8,8 +8,10 @@
 #include "gpu/dedisperser.hpp"
 #include "gpu/dedisperser_group.hpp"
 #include "gpu/kernels.hpp"
+#include "gpu/scorer.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -633,6 +635,109 @@ int pdd_group_stop(pdd_group *g, const char *reason)
     PDD_TRY({
         PDD_CHECK(g && g->g, "null group handle");
         g->g->stop(reason ? reason : "");
+    });
+}
+
+
+// ---- learned peak scorer ----
+
+static scorer::TileSource tile_source(const pdd_tile_source *s, const char *who)
+{
+    PDD_CHECK(s && s->base, who << ": null tile source");
+    PDD_CHECK(s->dtype >= 0 && s->dtype <= 2, who << ": tile dtype must be 0 (float32), 1 (float16) or 2 (bfloat16)");
+    PDD_CHECK(s->nbeams >= 0 && s->nrows >= 0 && s->beam_stride >= 0 && s->row_stride >= 0 && s->t_lo <= s->t_hi,
+              who << ": bad tile source extents");
+    scorer::TileSource t;
+    t.base = s->base;
+    t.dtype = s->dtype;
+    t.nbeams = s->nbeams;
+    t.beam_stride = s->beam_stride;
+    t.nrows = s->nrows;
+    t.row_stride = s->row_stride;
+    t.t_lo = s->t_lo;
+    t.t_hi = s->t_hi;
+    t.origins = s->origins;
+    return t;
+}
+
+static void launch_check(const char *who)
+{
+    hipError_t e = hipGetLastError();
+    PDD_CHECK(e == hipSuccess, who << ": launch failed: " << hipGetErrorString(e));
+}
+
+static constexpr int kScorerMaxParts = 512;
+
+int64_t pdd_scorer_nparams(void) { return scorer::kNParams; }
+
+int pdd_scorer_forward(const float *params, const pdd_tile_source *src, int64_t ntiles, float *logits, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(params && (logits || ntiles == 0), "pdd_scorer_forward: null argument");
+        PDD_CHECK(ntiles >= 0, "pdd_scorer_forward: ntiles < 0");
+        const scorer::TileSource t = tile_source(src, "pdd_scorer_forward");
+        scorer::launch_forward(params, t, ntiles, logits, (hipStream_t) stream);
+        launch_check("pdd_scorer_forward");
+    });
+}
+
+int pdd_scorer_grad(const float *params, const pdd_tile_source *src, const float *labels, int64_t ntiles,
+                    float *grad, float *logits, float *loss_terms, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(params && grad, "pdd_scorer_grad: null argument");
+        PDD_CHECK(ntiles >= 0, "pdd_scorer_grad: ntiles < 0");
+        PDD_CHECK(ntiles == 0 || (labels && logits && loss_terms), "pdd_scorer_grad: null argument");
+        const scorer::TileSource t = tile_source(src, "pdd_scorer_grad");
+        hipStream_t st = (hipStream_t) stream;
+        const int nparts = scorer::train_nparts(ntiles, kScorerMaxParts);
+        float *partials = nullptr;
+        if (nparts > 0)
+            hip_check(hipMallocAsync((void **) &partials, size_t(nparts) * scorer::kNParams * sizeof(float), st),
+                      "pdd_scorer_grad workspace");
+        scorer::launch_train(params, t, labels, ntiles, ntiles ? float(1.0 / double(ntiles)) : 0.0f, logits,
+                             loss_terms, partials, nparts, st);
+        scorer::launch_reduce(partials, nparts, grad, st);
+        launch_check("pdd_scorer_grad");
+        if (partials)
+            hip_check(hipFreeAsync(partials, st), "pdd_scorer_grad workspace");
+    });
+}
+
+int pdd_scorer_adam(float *params, float *m, float *v, const float *grad, int64_t step, float lr, float beta1,
+                    float beta2, float eps, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(params && m && v && grad, "pdd_scorer_adam: null argument");
+        PDD_CHECK(step >= 1, "pdd_scorer_adam: step counts from 1");
+        PDD_CHECK(beta1 >= 0 && beta1 < 1 && beta2 >= 0 && beta2 < 1 && eps > 0, "pdd_scorer_adam: bad hyperparameters");
+        const float bc1 = float(1.0 - std::pow(double(beta1), double(step)));
+        const float bc2 = float(1.0 - std::pow(double(beta2), double(step)));
+        scorer::launch_adam(params, m, v, grad, scorer::kNParams, lr, beta1, beta2, eps, bc1, bc2, (hipStream_t) stream);
+        launch_check("pdd_scorer_adam");
+    });
+}
+
+int pdd_dedisperser_tree_rows(pdd_dedisperser *dd, int64_t itree, pdd_tile_source *src)
+{
+    PDD_TRY({
+        PDD_CHECK(dd && dd->d && src, "null argument");
+        void *base = nullptr;
+        long nrows = 0;
+        long bstride = 0;
+        long rstride = 0;
+        long t_lo = 0;
+        long t_hi = 0;
+        dd->d->tree_rows(itree, base, nrows, bstride, rstride, t_lo, t_hi);
+        src->base = base;
+        src->dtype = dd->d->dtype == Dtype::float32 ? 0 : (dd->d->dtype == Dtype::float16 ? 1 : 2);
+        src->nbeams = dd->d->total_beams;
+        src->beam_stride = bstride;
+        src->nrows = nrows;
+        src->row_stride = rstride;
+        src->t_lo = t_lo;
+        src->t_hi = t_hi;
+        src->origins = nullptr;
     });
 }
 

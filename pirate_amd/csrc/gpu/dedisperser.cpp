@@ -1218,4 +1218,20 @@ size_t GpuDedisperser::debug_copy(const std::string &name, void *host, size_t ma
     return a->nbytes;
 }
 
+void GpuDedisperser::tree_rows(long itree, void *&base, long &nrows, long &beam_stride, long &row_stride, long &t_lo,
+                               long &t_hi)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    PDD_CHECK(is_allocated, "tree_rows(): not allocated");
+    PDD_CHECK(itree >= 0 && itree < ntrees, "tree_rows(): itree out of range");
+    const TreeTables &T = tables[itree];
+    const DedispersionTree &t = plan->trees[itree];
+    base = (char *) sb[itree].ptr + size_t(T.lead + T.tpad) * elt_bytes;
+    nrows = long(t.ndm_out) * T.M;
+    row_stride = T.rowlen;
+    beam_stride = nrows * T.rowlen;
+    t_lo = -T.tpad;
+    t_hi = t.nt_ds;
+}
+
 }  // namespace pirate_amd

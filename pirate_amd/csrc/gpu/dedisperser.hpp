@@ -124,6 +124,12 @@ public:
 
     // Copies an internal buffer to the host (tests): name in {"grid", "lds<p>", "sb<t>"}.
     size_t debug_copy(const std::string &name, void *host, size_t maxbytes);
+    // Peak-finder input rows of tree itree (the stage-2 output): row r = idm_out * M + m of global
+    // beam b, chunk sample t at base + (b * beam_stride + r * row_stride + t) * elt_bytes, valid
+    // for -tpad <= t < nt_ds (t < 0: the previous chunk's last samples). The rows hold the latest
+    // chunk whose stage 2 has run on the beam's batch.
+    void tree_rows(long itree, void *&base, long &nrows, long &beam_stride, long &row_stride, long &t_lo,
+                   long &t_hi);
 
     // Per-kernel timing with HIP events recorded on the compute stream around every kernel
     // launch (kernel names: gridding, lds, stage1_p<p>, stage2_t<i>, pf_t<i>). The report

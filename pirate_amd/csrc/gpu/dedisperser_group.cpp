@@ -16,6 +16,15 @@ void launch_advance_counter(long *counter, hipStream_t st);   // k_group.hip
 #define HIP_CHECK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
     throw pdd_error(std::string("HIP error: ") + hipGetErrorString(e_) + " in " #x); } while (0)
 
+// Graph replay is switched off: its first run on MI355X ended in a segmentation fault inside
+// release_input_and_launch (the capture / instantiate / launch of the members' chains), whose
+// cause has not been identified. The chunk's launches are issued one by one.
+static void refuse_graph()
+{
+    throw pdd_error("GpuDedisperserGroup: HIP graph replay is disabled in this build (a segmentation fault in "
+                    "its capture path is not yet explained); use use_graph = false");
+}
+
 
 GpuDedisperserGroup::GpuDedisperserGroup(const std::vector<std::shared_ptr<DedispersionPlan>> &plans,
                                          const GpuDedisperserGroupParams &params_) :
@@ -31,6 +40,8 @@ GpuDedisperserGroup::GpuDedisperserGroup(const std::vector<std::shared_ptr<Dedis
                   "GpuDedisperserGroup: every member needs the same chunk (time_samples_per_chunk, time_sample_ms)");
     }
     PDD_CHECK(params.nslots >= 1 && params.nbatches_out >= 1, "GpuDedisperserGroup: nslots and nbatches_out must be >= 1");
+    if (params.use_graph)
+        refuse_graph();
     nslots = params.nslots;
     for (size_t m = 0; m < plans.size(); m++) {
         GpuDedisperserParams dp;
@@ -164,6 +175,8 @@ void GpuDedisperserGroup::drop_graphs()
 void GpuDedisperserGroup::set_use_graph(bool on)
 {
     std::lock_guard<std::mutex> lk(mutex);
+    if (on)
+        refuse_graph();
     params.use_graph = on;
     drop_graphs();
 }

@@ -6,11 +6,11 @@
 // sequence, include/pirate/DedispersionConfig.hpp:81-84, src_lib/GpuDedisperser.cpp:988-1105).
 // Here every member is a GpuDedisperser whose whole beam set is one batch; per chunk, every
 // member's chain (gridding + lagged downsampling, stage-1 trees, stage-2 trees, peak finding)
-// is issued as independent branches of ONE HIP graph launch, so the members' kernels share the
-// device at once and a chunk costs one launch call. The graphs are captured once per
-// (input slot, output slot, weight slot) phase of the chunk index and replayed; the kernels
-// read the chunk index from a device counter that the graph's last node advances, so a
-// replayed graph needs no parameter updates.
+// is issued on a branch stream of its own, forked from and joined into one launch stream, so
+// the members' kernels share the device at once. The kernels can read the chunk index from a
+// device counter that the chunk's last node advances (for replay of a captured HIP graph, one
+// per (input slot, output slot, weight slot) phase); graph replay is refused in this build
+// (see dedisperser_group.cpp), and the launches are issued one by one.
 //
 // Per-chunk protocol (one producer, one consumer, chunks in order):
 //   acquire_input(c) -> per-member input slots (B_m, F_m, T) to fill on the caller's stream
@@ -36,7 +36,7 @@ struct GpuDedisperserGroupParams {
     long nslots = 2;         // input slots per member (the caller fills chunk c+1 while c runs)
     long nbatches_out = 2;   // output slots per member
     int device = 0;
-    bool use_graph = true;   // false: the same launches issued one by one (eager), for comparison
+    bool use_graph = false;  // true (HIP graph replay) is refused: see dedisperser_group.cpp
 };
 
 class GpuDedisperserGroup {

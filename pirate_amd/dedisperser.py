@@ -190,6 +190,17 @@ class GpuDedisperser:
         return ResourceTracker(self.plan, gmem_footprint=self.gpu_bytes(), hmem_footprint=self.host_bytes(),
                                memcpy_h2g=n.value, memcpy_g2h=n.value)
 
+    def tree_rows(self, itree):
+        """scorer.TileSource over the peak-finder input rows of tree itree (its stage-2 output):
+        row idm_out * M + m of every global beam, sample t of the latest chunk whose stage 2 ran
+        on the beam's batch (t_lo = -history, t_hi = the tree's chunk samples). With one
+        full-band subband, row r is DM trial r. Valid after get_output of that chunk and
+        before the batch's next launch. Add tile origins with .with_origins()."""
+        from .scorer import TileSource
+        s = L.TileSource()
+        L.check(L.lib().pdd_dedisperser_tree_rows(self._h, itree, ctypes.byref(s)))
+        return TileSource(s, [self])
+
     def time(self, niterations, log=print):
         """Timing loop of the reference's GpuDedisperser::time (src_lib/GpuDedisperser.cpp:1573-1690):
         every iteration uploads an int4 chunk and its float16 scales/offsets from pinned host

@@ -30,7 +30,7 @@ from .dedisperser import GpuDedisperser, GpuDedisperserOutputs, device_view, _st
 
 
 class GpuDedisperserGroup:
-    def __init__(self, plans, nslots=2, nbatches_out=2, cuda_device_id=0, use_graph=True):
+    def __init__(self, plans, nslots=2, nbatches_out=2, cuda_device_id=0, use_graph=False):
         plans = list(plans)
         if not plans:
             raise ValueError('GpuDedisperserGroup: no plans')

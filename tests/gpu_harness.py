@@ -89,11 +89,14 @@ def quantize(x, dtype):
     return torch.from_numpy(x).to(tdt).to(torch.float64).numpy()
 
 
-def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print):
+def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print, rfi=False):
     """Runs the GPU chain with 16-bit storage and the FP64 oracle on the same (quantized) inputs
     with bare-kernel weights (outputs in SNR units for unit-variance noise). Returns per-tree
     error statistics: max and rms of |gpu - fp64| / max(|fp64|, 1) over out_max, and the same
-    for the FP64 value of the GPU's argmax token (how far the chosen candidate is from the max)."""
+    for the FP64 value of the GPU's argmax token (how far the chosen candidate is from the max);
+    'by_trial': the max error per (DM quarter of the tree, peak-finding profile of the GPU's
+    argmax), i.e. per (DM, width) trial class. rfi=True adds narrowband RFI: 1% of the channels
+    of every beam (drawn once from the seed) carry a constant +5 (sigma) offset."""
     import torch
     import pirate_amd as pa
 
@@ -116,10 +119,15 @@ def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print):
     B, F, T = plan.beams_per_batch, plan.nfreq, plan.nt_in
     errs = {it: [] for it in range(plan.ntrees)}
     tok_errs = {it: [] for it in range(plan.ntrees)}
+    trial = {it: {} for it in range(plan.ntrees)}
+    offs = np.zeros((B, F, 1), dtype=np.float32)
+    if rfi:
+        for b in range(B):
+            offs[b, rng.choice(F, size=max(1, F // 100), replace=False), 0] = 5.0
     for ichunk in range(nchunks):
         for ib in range(dd.nbatches):
             seq = ichunk * dd.nbatches + ib
-            x = rng.standard_normal((B, F, T)).astype(np.float32)
+            x = rng.standard_normal((B, F, T)).astype(np.float32) + offs
             xq = quantize(x, dtype)
             with dd.get_input(seq) as arr:
                 arr.copy_(torch.from_numpy(xq).to(tdt).to(arr.device))
@@ -130,16 +138,28 @@ def run_precision(cfg_dict, dtype, nchunks, seed=0, log=print):
                     ga = out.out_argmax[it].cpu().numpy().view(np.uint32)
                     rm, _ = ref.output(it)
                     scale = np.maximum(np.abs(rm), 1.0)
-                    errs[it].append((np.abs(gm - rm) / scale).ravel())
+                    e = np.abs(gm - rm) / scale
+                    errs[it].append(e.ravel())
+                    p = (ga >> 8) & 0xff
+                    q = (np.arange(e.shape[1]) * 4 // e.shape[1])[None, :, None]
+                    for qq in range(4):
+                        for pp in np.unique(p):
+                            m = (np.broadcast_to(q, e.shape) == qq) & (p == pp)
+                            if m.any():
+                                k = (qq, int(pp))
+                                trial[it][k] = max(trial[it].get(k, 0.0), float(e[m].max()))
                     ev = ref.eval_tokens(it, ga)
                     tok_errs[it].append((np.abs(ev - rm) / scale).ravel())
     stats = {}
     for it in range(plan.ntrees):
         e = np.concatenate(errs[it])
         te = np.concatenate(tok_errs[it])
-        stats[it] = dict(max=float(e.max()), rms=float(np.sqrt(np.mean(e**2))), tok_max=float(te.max()), n=int(e.size))
+        stats[it] = dict(max=float(e.max()), rms=float(np.sqrt(np.mean(e**2))), tok_max=float(te.max()), n=int(e.size),
+                         by_trial=dict(sorted(trial[it].items())))
         log(f'{dtype} tree {it}: max rel err {stats[it]["max"]:.2e}, rms {stats[it]["rms"]:.2e}, '
             f'argmax candidate max rel err {stats[it]["tok_max"]:.2e} over {e.size} (dm, t) outputs')
+        log(f'{dtype} tree {it}: max rel err per (DM quarter, profile): '
+            + ', '.join(f'{k}: {v:.1e}' for k, v in stats[it]['by_trial'].items()))
     return stats
 
 

@@ -27,7 +27,7 @@ ROOT = os.path.dirname(HERE)
 def test_library_exports_every_declared_symbol():
     with open(os.path.join(ROOT, 'include', 'pirate_amd.h')) as f:
         text = f.read()
-    declared = set(re.findall(r'^\s*(?:int|void|const char)\s*\*?\s*(pdd_\w+)\s*\(', text, re.M))
+    declared = set(re.findall(r'^\s*(?:int64_t|int|void|const char)\s*\*?\s*(pdd_\w+)\s*\(', text, re.M))
     assert len(declared) >= 25
     L = _lib.lib()
     for name in sorted(declared):
@@ -284,3 +284,12 @@ def test_device_view_keeps_its_owner_alive():
     del v
     gc.collect()
     assert ref() is None
+
+
+def test_group_refuses_graph_replay():
+    """HIP graph replay of a group's chunk is refused (its capture path is switched off): the
+    group is created with use_graph=False by default, and use_graph=True fails at creation."""
+    import pirate_amd as pa
+    plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(CASES['one_tree']))
+    with pytest.raises(RuntimeError, match='graph replay is disabled'):
+        pa.GpuDedisperserGroup([plan], use_graph=True)

@@ -17,9 +17,9 @@ from gpu_cases import CASES
 pytestmark = pytest.mark.gpu
 
 
-def make(name='toy', **kw):
+def make(name='toy', overrides=None, **kw):
     import pirate_amd as pa
-    cfg = pa.DedispersionConfig.from_dict(CASES[name])
+    cfg = pa.DedispersionConfig.from_dict(dict(CASES[name], **(overrides or {})))
     plan = pa.DedispersionPlan(cfg)
     dd = pa.GpuDedisperser(plan, **kw)
     dd.allocate()
@@ -73,7 +73,8 @@ def test_negative_weights_rejected():
 
 def test_two_consumers_and_back_pressure():
     import torch
-    dd = make('toy', num_consumers=2, nbatches_out=1)
+    # one active batch: nbatches_out may not be below num_active_batches (as in the reference)
+    dd = make('toy', overrides=dict(num_active_batches=1), num_consumers=2, nbatches_out=1)
     push(dd, 0)
     dd._acquire_input(1, 0)
     res = {'done': False, 'err': None}
