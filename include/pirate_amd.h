@@ -343,6 +343,16 @@ int pdd_scorer_forward(const float *params, const pdd_tile_source *src, int64_t 
 int pdd_scorer_grad(const float *params, const pdd_tile_source *src, const float *labels, int64_t ntiles,
                     float *grad, float *logits, float *loss_terms, void *stream);
 
+/* pdd_scorer_grad without allocation. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc). Same arguments and results, bit for bit, as pdd_scorer_grad; it calls no
+ * hipMalloc / hipFree. The per-work-group partial gradients live in a file-scope device array
+ * of 512 x 13905 floats (28.5 MB per GPU, held until the process ends), which covers any
+ * ntiles (a work group accumulates several tiles). Calls that overlap on one GPU race on that
+ * array: serialize them (one stream, or events). */
+int pdd_scorer_grad_noalloc(const float *params, const pdd_tile_source *src, const float *labels, int64_t ntiles,
+                            float *grad, float *logits, float *loss_terms, void *stream);
+
 /* One Adam step (step counted from 1) on params, with moment estimates m and v (nparams floats
  * each, zero before step 1). */
 int pdd_scorer_adam(float *params, float *m, float *v, const float *grad, int64_t step, float lr, float beta1,

@@ -106,6 +106,7 @@ _SIGS = {
     'pdd_scorer_nparams': (i64, []),
     'pdd_scorer_forward': (ctypes.c_int, [vp, ctypes.POINTER(TileSource), i64, vp, vp]),
     'pdd_scorer_grad': (ctypes.c_int, [vp, ctypes.POINTER(TileSource), vp, i64, vp, vp, vp, vp]),
+    'pdd_scorer_grad_noalloc': (ctypes.c_int, [vp, ctypes.POINTER(TileSource), vp, i64, vp, vp, vp, vp]),
     'pdd_scorer_adam': (ctypes.c_int, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, vp]),
     'pdd_dedisperser_tree_rows': (ctypes.c_int, [vp, i64, ctypes.POINTER(TileSource)]),

@@ -666,7 +666,7 @@ static void launch_check(const char *who)
     PDD_CHECK(e == hipSuccess, who << ": launch failed: " << hipGetErrorString(e));
 }
 
-static constexpr int kScorerMaxParts = 512;
+static constexpr int kScorerMaxParts = scorer::kScratchParts;   // same partial count as the no-allocation variant
 
 int64_t pdd_scorer_nparams(void) { return scorer::kNParams; }
 
@@ -701,6 +701,20 @@ int pdd_scorer_grad(const float *params, const pdd_tile_source *src, const float
         launch_check("pdd_scorer_grad");
         if (partials)
             hip_check(hipFreeAsync(partials, st), "pdd_scorer_grad workspace");
+    });
+}
+
+int pdd_scorer_grad_noalloc(const float *params, const pdd_tile_source *src, const float *labels, int64_t ntiles,
+                            float *grad, float *logits, float *loss_terms, void *stream)
+{
+    PDD_TRY({
+        PDD_CHECK(params && grad, "pdd_scorer_grad_noalloc: null argument");
+        PDD_CHECK(ntiles >= 0, "pdd_scorer_grad_noalloc: ntiles < 0");
+        PDD_CHECK(ntiles == 0 || (labels && logits && loss_terms), "pdd_scorer_grad_noalloc: null argument");
+        const scorer::TileSource t = tile_source(src, "pdd_scorer_grad_noalloc");
+        scorer::grad_noalloc(params, t, labels, ntiles, ntiles ? float(1.0 / double(ntiles)) : 0.0f, grad, logits,
+                             loss_terms, (hipStream_t) stream);
+        launch_check("pdd_scorer_grad_noalloc");
     });
 }
 

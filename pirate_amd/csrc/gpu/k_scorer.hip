@@ -20,6 +20,9 @@
 #include "device_common.hpp"
 #include "scorer.hpp"
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
 
 namespace pirate_amd {
 namespace scorer {
@@ -570,6 +573,51 @@ void launch_adam(float *params, float *m, float *v, const float *grad, long n, f
                        eps, bc1, bc2);
 }
 
+
+// ---- gradient without allocation ----
+// The partial gradients of pdd_scorer_grad_noalloc live in this file-scope device array (one
+// copy per GPU, held until the process ends): kScratchParts work groups of kNParams floats,
+// the same partial count as the allocating gradient's, so both give identical results. Every
+// call overwrites all the partials it reduces, so nothing needs clearing; calls that overlap
+// on one GPU race on the array.
+static __device__ __align__(256) float scratch[size_t(kScratchParts) * kNParams];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static float *g_scratch_ptr[kMaxDevices];        // [device]: address of that device's array
+
+// The array's address on the current device, looked up once per device (with that device
+// current) by hipGetSymbolAddress; host code uses only this address, never the array's name.
+static float *scratch_address()
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("scorer scratch: hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        throw std::runtime_error("scorer scratch: device index " + std::to_string(dev) + " out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("scorer scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+void grad_noalloc(const float *params, const TileSource &src, const float *labels, long ntiles, float inv_n,
+                  float *grad, float *logits, float *loss_terms, hipStream_t st)
+{
+    float *partials = scratch_address();
+    const int nparts = train_nparts(ntiles, kScratchParts);
+    if (nparts > kScratchParts)
+        throw std::runtime_error("scorer gradient without allocation: " + std::to_string(nparts)
+                                 + " partial gradients exceed the " + std::to_string(kScratchParts) + " of the scratch array");
+    launch_train(params, src, labels, ntiles, inv_n, logits, loss_terms, partials, nparts, st);
+    launch_reduce(partials, nparts, grad, st);
+}
 
 }  // namespace scorer
 }  // namespace pirate_amd

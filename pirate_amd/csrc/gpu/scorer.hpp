@@ -62,6 +62,15 @@ void launch_train(const float *params, const TileSource &src, const float *label
 // grad[p] = sum_{j < nparts} partials[j, p], in order of j.
 void launch_reduce(const float *partials, int nparts, float *grad, hipStream_t st);
 
+// The gradient above without allocation: partial gradients in a file-scope device array of
+// kScratchParts x kNParams floats per GPU (28.5 MB), whose address comes from
+// hipGetSymbolAddress on the first call on each device. Identical results to
+// launch_train + launch_reduce with nparts = train_nparts(ntiles, kScratchParts); overlapping
+// calls on one GPU race on the array.
+constexpr int kScratchParts = 512;
+void grad_noalloc(const float *params, const TileSource &src, const float *labels, long ntiles, float inv_n,
+                  float *grad, float *logits, float *loss_terms, hipStream_t st);
+
 // Adam (Kingma & Ba 2015, Algorithm 1) on n float32 parameters; bc1 = 1 - beta1^step,
 // bc2 = 1 - beta2^step.
 void launch_adam(float *params, float *m, float *v, const float *grad, long n, float lr, float beta1, float beta2,

@@ -145,6 +145,21 @@ class LearnedScorer:
                                         _stream_ptr(stream)))
         return self.grad_buf, logits, loss
 
+    def grad_noalloc(self, src, labels, grad, logits, loss_terms, stream=None):
+        """grad() through pdd_scorer_grad_noalloc, which allocates nothing: the caller passes
+        the label vector and the outputs (float32 device tensors of NPARAMS, n, n and n
+        elements); the partial gradients live in the library's fixed device array (calls that
+        overlap on one GPU race on it). Results equal grad()'s bit for bit."""
+        s = self._source(src)
+        for t, n in ((labels, s.ntiles), (grad, NPARAMS), (logits, s.ntiles), (loss_terms, s.ntiles)):
+            if not (t.is_cuda and t.is_contiguous() and t.dtype == self.params.dtype and t.numel() == n):
+                raise ValueError('LearnedScorer.grad_noalloc: labels and outputs must be contiguous float32 '
+                                 'device tensors of the right sizes')
+        L.check(L.lib().pdd_scorer_grad_noalloc(ctypes.c_void_p(self.params.data_ptr()), ctypes.byref(s.struct),
+                                                ctypes.c_void_p(labels.data_ptr()), s.ntiles,
+                                                ctypes.c_void_p(grad.data_ptr()), ctypes.c_void_p(logits.data_ptr()),
+                                                ctypes.c_void_p(loss_terms.data_ptr()), _stream_ptr(stream)))
+
     def adam_step(self, grad=None, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
         g = self.grad_buf if grad is None else grad
         self.step += 1
