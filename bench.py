@@ -193,6 +193,8 @@ def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
     log(f'{plan.dtype}: allocated {gib:.2f} GiB of device memory, {dd.host_bytes() / 2**30:.2f} GiB of pinned host memory')
     for it in range(plan.ntrees):
         dd.fill_host_weights(it)          # bare-kernel weights: out_max in SNR units for unit-variance noise
+    tuned = dd.apply_tuning_table()      # launch shapes recorded by python -m pirate_amd.autotune
+    log(f'{plan.dtype}: launch shapes {tuned if tuned else "default (no tuning-table entry)"}')
 
     seq = 0
     with dd.get_input(seq) as arr:
@@ -222,6 +224,7 @@ def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
     log(f'{plan.dtype}: {steps} steps, {dt * 1e3:.3f} ms/step')
 
     # Per-kernel times: HIP events on the stream each kernel runs on, every kernel alone.
+    fork0 = dd.get_tuning('fork')
     dd.set_tuning('fork', 2)
     dd.set_kernel_timing(True)
     for _ in range(timing_steps):
@@ -229,7 +232,7 @@ def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
     torch.cuda.synchronize()
     kt = dd.kernel_timing()
     dd.set_kernel_timing(False)
-    dd.set_tuning('fork', -1)
+    dd.set_tuning('fork', fork0)
     return plan, dt, per_kernel_table(plan, kt), gib, dd
 
 
@@ -247,6 +250,9 @@ def measure_batched(pa, torch, steps, warmup, timing_steps):
     for m in g.members:
         for it in range(m.ntrees):
             m.fill_host_weights(it)
+        tuned = m.apply_tuning_table()
+        log(f'fp16 batched member rank {m.plan.config.toplevel_tree_rank}: launch shapes '
+            f'{tuned if tuned else "default (no tuning-table entry)"}')
     ichunk = 0
     for _ in range(2):
         with g.get_input(ichunk) as slots:      # both input slots hold a resident chunk
@@ -278,18 +284,17 @@ def measure_batched(pa, torch, steps, warmup, timing_steps):
     dt = timed(steps)
     log(f'fp16 batched: {dt * 1e3:.3f} ms/step')
 
-    # Per-kernel: every member's kernels alone on one stream (eager while timing).
+    # Per-kernel: each member alone (the other members' launches paused), its kernels one at a
+    # time on one stream, timed with HIP events. The group issues every member per chunk, so
+    # the timing runs on standalone dedispersers of the same plans.
     per_member = []
-    for m in g.members:
-        m.set_tuning('fork', 2)
-        m.set_kernel_timing(True)
-    for _ in range(timing_steps):
-        step()
+    del g
     torch.cuda.synchronize()
-    for mi, (m, p) in enumerate(zip(g.members, plans)):
-        per_member.append(per_kernel_table(p, m.kernel_timing()))
-        m.set_kernel_timing(False)
-        m.set_tuning('fork', -1)
+    for d in cfgs:
+        _, _, pk, _, dd = measure(pa, torch, d, 2, 1, timing_steps)
+        per_member.append(pk)
+        del dd
+        torch.cuda.synchronize()
     serial_ms = sum(v['avg_ms'] for pk in per_member for v in pk.values())
     alg = sum(chain_alg_bytes(p) for p in plans)
     kern_bytes = sum(v['alg_bytes'] for pk in per_member for v in pk.values())

@@ -330,6 +330,19 @@ class GpuDedisperser:
         'grid_xcd_run' or 'fork' (see include/pirate_amd.h); applies from the next launch."""
         L.check(L.lib().pdd_dedisperser_set_tuning(self._h, name.encode(), int(value)))
 
+    def apply_tuning_table(self, table=None):
+        """Sets the launch shapes that pirate_amd.autotune recorded for this dedisperser's
+        workload signature (tree rank, channels, primary trees, dtype, beams, chunk); returns
+        them, or None (defaults kept) when the table has no entry for it."""
+        from . import autotune
+        t = autotune.load_table() if table is None else table
+        ent = t.get(autotune.signature(self.plan))
+        if ent is None:
+            return None
+        for k, v in ent['tuning'].items():
+            self.set_tuning(k, v)
+        return dict(ent['tuning'])
+
     def get_tuning(self, name):
         v = ctypes.c_int64()
         L.check(L.lib().pdd_dedisperser_get_tuning(self._h, name.encode(), ctypes.byref(v)))

@@ -7,7 +7,8 @@ Data: pirate_amd.scorer_data (the 1024-channel rank-10 chain, 16 beams per chunk
 pulses in half of the beams, noise + RFI everywhere). Training and evaluation use disjoint
 pulse banks and noise streams. Prints progress lines and one JSON summary: training loss per
 epoch, completeness of the scorer and of the peak finder at false-alarm rates 1e-2, 1e-3 and
-1e-4 on the evaluation negatives (overall and per SNR bin), the data-generation rate, and the
+1e-4 on evaluation negatives with RFI and, separately, on pure-noise evaluation data (every
+beam without RFI), overall and per SNR bin, the data-generation rate, and the
 scorer kernels' throughput measured with HIP events (forward: tiles/s and bf16 MFMA TFLOP/s at
 2 * 16 * 256 * (64 + 16 + 16) * 9 flop per tile; training step: tiles/s).
 """
@@ -65,6 +66,7 @@ def main(argv=None):
     ap.add_argument('--eval-bank', type=int, default=128)
     ap.add_argument('--eval-chunks', type=int, default=400)
     ap.add_argument('--seed', type=int, default=5)
+    ap.add_argument('--nproc', type=int, default=None, help='worker processes for the pulse bank (1: none)')
     ap.add_argument('--out', default=None, help='save the trained parameters (.npz)')
     ap.add_argument('--json', default=None, help='also write the summary here')
     a = ap.parse_args(argv)
@@ -73,8 +75,8 @@ def main(argv=None):
     import pirate_amd as pa
     from . import scorer_data as sd
     t0 = time.perf_counter()
-    bank = sd.make_template_bank(a.bank, seed=1000 * a.seed + 1)
-    ebank = sd.make_template_bank(a.eval_bank, seed=1000 * a.seed + 2)
+    bank = sd.make_template_bank(a.bank, seed=1000 * a.seed + 1, nproc=a.nproc)
+    ebank = sd.make_template_bank(a.eval_bank, seed=1000 * a.seed + 2, nproc=a.nproc)
     t_bank = time.perf_counter() - t0
     log(f'{len(bank)} + {len(ebank)} simulated pulses in {t_bank:.1f} s')
 
@@ -97,25 +99,35 @@ def main(argv=None):
     log(f'kernels: {rates}')
     del train
 
+    def evaluate(ev, what):
+        s = sd.score(scr, ev['tiles'])
+        y = ev['labels'].cpu().numpy() > 0.5
+        res = {'pulses': int(y.sum()), 'negatives': int((~y).sum()), 'far': {}}
+        for far in (1e-2, 1e-3, 1e-4):
+            c_nn, thr_nn = sd.completeness_at_far(s[y], s[~y], far)
+            c_pf, thr_pf = sd.completeness_at_far(ev['pf'][y], ev['pf'][~y], far)
+            bins = {}
+            for lo, hi in ((5, 10), (10, 15), (15, 20), (20, 30)):
+                m = (ev['snr'][y] >= lo) & (ev['snr'][y] < hi)
+                bins[f'snr_{lo}_{hi}'] = {'scorer': float(np.mean(s[y][m] > thr_nn)),
+                                          'peak_finder': float(np.mean(ev['pf'][y][m] > thr_pf)), 'n': int(m.sum())}
+            res['far'][f'{far:g}'] = {'scorer': c_nn, 'peak_finder': c_pf, 'scorer_threshold_logit': thr_nn,
+                                      'peak_finder_threshold_snr': thr_pf, 'by_snr': bins}
+            log(f'{what}: FAR {far:g}: completeness scorer {c_nn:.3f}, peak finder {c_pf:.3f} '
+                f'(peak-finder threshold SNR {thr_pf:.1f})')
+        return res
+
     ev = sd.build_dataset(sd.TileStream(ebank, nbeams=16, seed=a.seed + 1, neg_per_beam=128), a.eval_chunks, log=log)
-    s = sd.score(scr, ev['tiles'])
-    y = ev['labels'].cpu().numpy() > 0.5
-    res = {'far': {}}
-    for far in (1e-2, 1e-3, 1e-4):
-        c_nn, thr_nn = sd.completeness_at_far(s[y], s[~y], far)
-        c_pf, thr_pf = sd.completeness_at_far(ev['pf'][y], ev['pf'][~y], far)
-        bins = {}
-        for lo, hi in ((5, 10), (10, 15), (15, 20), (20, 30)):
-            m = (ev['snr'][y] >= lo) & (ev['snr'][y] < hi)
-            bins[f'snr_{lo}_{hi}'] = {'scorer': float(np.mean(s[y][m] > thr_nn)),
-                                      'peak_finder': float(np.mean(ev['pf'][y][m] > thr_pf)), 'n': int(m.sum())}
-        res['far'][f'{far:g}'] = {'scorer': c_nn, 'peak_finder': c_pf, 'scorer_threshold_logit': thr_nn,
-                                  'peak_finder_threshold_snr': thr_pf, 'by_snr': bins}
-        log(f'FAR {far:g}: completeness scorer {c_nn:.3f}, peak finder {c_pf:.3f}')
+    res_rfi = evaluate(ev, 'noise + RFI')
+    del ev
+    ev = sd.build_dataset(sd.TileStream(ebank, nbeams=16, seed=a.seed + 2, neg_per_beam=128, rfi=False),
+                          a.eval_chunks, log=log)
+    res_clean = evaluate(ev, 'noise only')
+    del ev
     summary = {'train_tiles': n, 'epochs': a.epochs, 'batch': a.batch, 'lr': a.lr, 'loss_per_epoch': hist,
-               'eval_pulses': int(y.sum()), 'eval_negatives': int((~y).sum()), 'bank': len(bank),
-               'eval_bank': len(ebank), 'bank_s': t_bank, 'data_s': t_data, 'data_tiles_per_s': n / t_data,
-               'train_s': t_train, 'kernels': rates, **res}
+               'bank': len(bank), 'eval_bank': len(ebank), 'bank_s': t_bank, 'data_s': t_data,
+               'data_tiles_per_s': n / t_data, 'train_s': t_train, 'kernels': rates,
+               'eval_noise_rfi': res_rfi, 'eval_noise_only': res_clean}
     if a.out:
         np.savez(a.out, params=scr.params.cpu().numpy())
     print(json.dumps(summary), flush=True)

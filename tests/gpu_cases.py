@@ -70,6 +70,16 @@ CASES = {
     'rank16_sb': dict(BASE, zone_nfreq=[2048], zone_freq_edges=[400, 800], toplevel_tree_rank=16,
                       time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                       frequency_subband_counts=[5, 9, 7, 3, 1], primary_trees=[pt(0, 8, 64, 64), pt(0, 4, 64, 64)]),
+    # Five and six primary trees: the gridding kernel's downsampling groups of 16 and 32
+    # samples leave 2 and 1 tiles per channel pair, so one workgroup takes 2 and 4 pairs.
+    'pri5': dict(BASE, zone_nfreq=[300], zone_freq_edges=[400, 800], toplevel_tree_rank=8,
+                 time_samples_per_chunk=2048, beams_per_gpu=2, beams_per_batch=2, num_active_batches=1,
+                 frequency_subband_counts=[1], primary_trees=[pt(0, 16, 16, 16), pt(0, 8, 16, 16), pt(0, 4, 16, 16),
+                                                              pt(0, 2, 16, 16), pt(0, 1, 16, 16)]),
+    'pri6': dict(BASE, zone_nfreq=[200], zone_freq_edges=[400, 800], toplevel_tree_rank=7,
+                 time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                 frequency_subband_counts=[1], primary_trees=[pt(0, 8, 16, 16), pt(0, 8, 16, 16), pt(0, 4, 16, 16),
+                                                              pt(0, 2, 16, 16), pt(0, 2, 16, 16), pt(0, 1, 16, 16)]),
     # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
     'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                  time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
@@ -97,8 +107,8 @@ def cfg2_with_beams(nbeams, batch=None):
 
 # Batched heterogeneous beams: three beam groups of 32 beams at tree ranks 12/13/14
 # (4096/8192/16384 channels over 300-1500 MHz) with 3/4/5 primary trees (downsampling levels),
-# float16 storage, chunks of 2048 samples, run as one GpuDedisperserGroup (one graph launch per
-# chunk). Not fixed by the workload and chosen here: no subband search ([1]), max_width 16 ms
+# float16 storage, chunks of 2048 samples, run as one GpuDedisperserGroup (each group's chain on
+# a branch stream of its own per chunk). Not fixed by the workload and chosen here: no subband search ([1]), max_width 16 ms
 # at every downsampling level, weight downsampling 64x64.
 def cfg4_group(rank, nbeams, dtype='float16'):
     nfreq = {12: 4096, 13: 8192, 14: 16384}[rank]
