@@ -359,11 +359,23 @@ int pdd_scorer_adam(float *params, float *m, float *v, const float *grad, int64_
                     float beta2, float eps, void *stream);
 
 /* Tile source over the peak-finder input rows of tree itree (its stage-2 output): rows
- * idm_out * M + m of every global beam, t = chunk sample (t_lo = -history samples, t_hi = chunk
- * samples of the tree). With a single full-band subband (M = 2^pf_rank multiplets), row r is
- * DM trial r of the tree. The rows hold the latest chunk whose stage 2 has run on the beam's
- * batch: read them after acquire_output of that chunk and before the batch's next launch. */
+ * idm_out * M + m of every global beam, t = chunk sample of the tree, t_lo <= t < t_hi = chunk
+ * samples (t_lo = the largest subband final lag, 0 without subbands: the peak finder moves the
+ * samples before it). With a single full-band subband (M = 2^pf_rank multiplets), row r is DM
+ * trial r of the tree. The rows hold the latest chunk of the beam's batch: read them after
+ * acquire_output of that chunk and before the batch's next launch. */
 int pdd_dedisperser_tree_rows(pdd_dedisperser *dd, int64_t itree, pdd_tile_source *src);
+
+/* The learned scorer inside the chain: from the next launch on, each chunk's rows of tree
+ * itree (full-band DM trials; rows a multiple of 64, chunk samples a multiple of 256) are cut
+ * into tiles of 64 rows x 256 samples on a fixed grid and scored right after the tree's stage
+ * 2, on the same stream. params: the scorer's pdd_scorer_nparams() float32 parameters in host
+ * memory (copied). Calling it again replaces the parameters. */
+int pdd_dedisperser_set_scorer(pdd_dedisperser *dd, int64_t itree, const float *params, int64_t nparams);
+
+/* The (B, ntiles_dm, ntiles_t) float32 logits of an acquired output seq_id (tile (b, i, j)
+ * covers rows 64 i .. 64 i + 63 and samples 256 j .. 256 j + 255 of beam b). */
+int pdd_output_scores(pdd_dedisperser *dd, int64_t seq_id, float **scores, int64_t *ntiles_dm, int64_t *ntiles_t);
 
 #ifdef __cplusplus
 }

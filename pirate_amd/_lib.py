@@ -110,6 +110,8 @@ _SIGS = {
     'pdd_scorer_adam': (ctypes.c_int, [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, vp]),
     'pdd_dedisperser_tree_rows': (ctypes.c_int, [vp, i64, ctypes.POINTER(TileSource)]),
+    'pdd_dedisperser_set_scorer': (ctypes.c_int, [vp, i64, ctypes.POINTER(ctypes.c_float), i64]),
+    'pdd_output_scores': (ctypes.c_int, [vp, i64, ctypes.POINTER(vp), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
 }
 
 EXPORTED_SYMBOLS = sorted(_SIGS)

@@ -732,6 +732,23 @@ int pdd_scorer_adam(float *params, float *m, float *v, const float *grad, int64_
     });
 }
 
+int pdd_dedisperser_set_scorer(pdd_dedisperser *dd, int64_t itree, const float *params, int64_t nparams)
+{
+    PDD_TRY_DD(dd, dd->d->set_scorer(itree, params, nparams));
+}
+
+int pdd_output_scores(pdd_dedisperser *dd, int64_t seq_id, float **scores, int64_t *ntiles_dm, int64_t *ntiles_t)
+{
+    PDD_TRY_DD(dd, {
+        PDD_CHECK(scores && ntiles_dm && ntiles_t, "null argument");
+        long a = 0;
+        long b = 0;
+        dd->d->output_scores(seq_id, *scores, a, b);
+        *ntiles_dm = a;
+        *ntiles_t = b;
+    });
+}
+
 int pdd_dedisperser_tree_rows(pdd_dedisperser *dd, int64_t itree, pdd_tile_source *src)
 {
     PDD_TRY({

@@ -1,4 +1,5 @@
 #include "dedisperser.hpp"
+#include "scorer.hpp"
 #include "../host/common.hpp"
 #include "../host/pf_variance.hpp"
 
@@ -84,6 +85,9 @@ GpuDedisperser::~GpuDedisperser()
                     &d_lvl0, &d_fl, &d_m_to_n, &d_stag_begin, &d_stag_goff, &d_out_stream, &s1_state, &s2_state, &weights, &out_var})
         for (auto &a : *v)
             freeit(a);
+    freeit(scorer_params);
+    freeit(scorer_origins);
+    freeit(scorer_out);
     freeit(out_region);     // out_max / out_argmax are views into it
     freeit(split_x);
     for (auto &t : timed_pending) {
@@ -765,6 +769,26 @@ void GpuDedisperser::launch_chain(const void *input, long ichunk, long ibatch, l
             timed_end(nm2, sp, ev2);
             if (late_fork && !serial && it == 0)
                 HIP_CHECK(hipEventRecord(ev_fork_s, st));
+            if (it == scorer_tree) {
+                // the learned scorer reads this chunk's rows before the peak finder rotates them
+                scorer::TileSource src;
+                src.base = (char *) sb[it].ptr + (size_t(T.lead + T.tpad) + size_t(beam0) * s2.sb_bstride) * elt_bytes;
+                src.dtype = (dtype == Dtype::float32) ? scorer::kFloat32 : (dtype == Dtype::float16) ? scorer::kFloat16
+                                                                                                     : scorer::kBfloat16;
+                src.nbeams = B;
+                src.beam_stride = s2.sb_bstride;
+                src.nrows = long(t.ndm_out) * T.M;
+                src.row_stride = T.rowlen;
+                src.t_lo = 0;
+                src.t_hi = t.nt_ds;
+                src.origins = (const int *) scorer_origins.ptr;
+                const long ntiles = B * scorer_ntr * scorer_ntt;
+                hipEvent_t ev3;
+                timed_begin("scorer_t" + std::to_string(it), sp, ev3);
+                scorer::launch_forward((const float *) scorer_params.ptr, src, ntiles,
+                                       (float *) scorer_out.ptr + oslot * ntiles, sp);
+                timed_end("scorer_t" + std::to_string(it), sp, ev3);
+            }
 
             PfArgs f;
             f.sb = (char *) sb[it].ptr + size_t(T.lead) * elt_bytes;
@@ -1218,6 +1242,65 @@ size_t GpuDedisperser::debug_copy(const std::string &name, void *host, size_t ma
     return a->nbytes;
 }
 
+void GpuDedisperser::set_scorer(long itree, const float *params_host, long nparams)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    PDD_CHECK(is_allocated, "set_scorer(): not allocated");
+    PDD_CHECK(itree >= 0 && itree < ntrees, "set_scorer(): itree out of range");
+    PDD_CHECK(params_host && nparams == scorer::kNParams, "set_scorer(): need " << scorer::kNParams << " parameters");
+    const DedispersionTree &t = plan->trees[itree];
+    const TreeTables &T = tables[itree];
+    const long nrows = long(t.ndm_out) * T.M;
+    PDD_CHECK(T.N == 1 && T.flmax == 0, "set_scorer(): tree " << itree << " has a subband search; the scorer reads "
+              "full-band DM trials");
+    PDD_CHECK(nrows % scorer::kC0 == 0 && t.nt_ds % scorer::kT == 0, "set_scorer(): tree " << itree << " has " << nrows
+              << " rows and " << t.nt_ds << " samples per chunk; tiles need multiples of 64 and 256");
+    for (hipStream_t x : compute_streams)
+        HIP_CHECK(hipStreamSynchronize(x));
+    const long B = beams_per_batch, ntr = nrows / scorer::kC0, ntt = t.nt_ds / scorer::kT;
+    if (!scorer_params.ptr)
+        scorer_params = dalloc(size_t(scorer::kNParams) * sizeof(float));
+    HIP_CHECK(hipMemcpy(scorer_params.ptr, params_host, size_t(scorer::kNParams) * sizeof(float), hipMemcpyHostToDevice));
+    if (scorer_tree != itree) {
+        std::vector<int> o;
+        o.reserve(size_t(B * ntr * ntt * 3));
+        for (long b = 0; b < B; b++)
+            for (long r = 0; r < ntr; r++)
+                for (long c = 0; c < ntt; c++) {
+                    o.push_back(int(b));
+                    o.push_back(int(r * scorer::kC0));
+                    o.push_back(int(c * scorer::kT));
+                }
+        for (DeviceArray *a : {&scorer_origins, &scorer_out})
+            if (a->ptr) {
+                HIP_CHECK(hipFree(a->ptr));
+                gpu_bytes_allocated -= a->nbytes;
+                a->ptr = nullptr;
+            }
+        scorer_origins = dalloc(o.size() * sizeof(int));
+        HIP_CHECK(hipMemcpy(scorer_origins.ptr, o.data(), o.size() * sizeof(int), hipMemcpyHostToDevice));
+        scorer_out = dalloc(size_t(nbatches_out) * B * ntr * ntt * sizeof(float));
+        scorer_ntr = ntr;
+        scorer_ntt = ntt;
+        scorer_tree = itree;
+    }
+}
+
+void GpuDedisperser::output_scores(long seq_id, float *&scores, long &ntiles_dm, long &ntiles_t)
+{
+    std::lock_guard<std::mutex> lk(mutex);
+    check_stopped_locked();
+    PDD_CHECK(scorer_tree >= 0, "output_scores(): no scorer set (set_scorer)");
+    bool held = false;
+    for (long c = 0; c < num_consumers; c++)
+        held = held || (seq_id >= curr_output_release[c] && seq_id < curr_output_acquire[c]);
+    PDD_CHECK(held, "output_scores(): seq_id " << seq_id << " is not an acquired output");
+    const long oslot = seq_id % nbatches_out;
+    scores = (float *) scorer_out.ptr + oslot * beams_per_batch * scorer_ntr * scorer_ntt;
+    ntiles_dm = scorer_ntr;
+    ntiles_t = scorer_ntt;
+}
+
 void GpuDedisperser::tree_rows(long itree, void *&base, long &nrows, long &beam_stride, long &row_stride, long &t_lo,
                                long &t_hi)
 {
@@ -1230,7 +1313,9 @@ void GpuDedisperser::tree_rows(long itree, void *&base, long &nrows, long &beam_
     nrows = long(t.ndm_out) * T.M;
     row_stride = T.rowlen;
     beam_stride = nrows * T.rowlen;
-    t_lo = -T.tpad;
+    // the peak finder ends the chunk by moving its last tpad + flmax samples to the row
+    // start, so [flmax, nt_ds) is what still holds this chunk's samples
+    t_lo = T.flmax;
     t_hi = t.nt_ds;
 }
 

@@ -90,6 +90,15 @@ public:
     // acquired (not yet released) seq_id; requires enable_variances.
     void output_variances(long seq_id, std::vector<double *> &out_var);
 
+    // Learned peak scorer on tree itree's rows, run on each chunk right after the tree's
+    // stage 2: the tree's rows (DM trials, with one full-band subband) are cut into tiles of
+    // 64 rows x 256 chunk samples on a fixed grid, and their logits go to an output ring of
+    // (nbatches_out, B, nrows / 64, nt_ds / 256) floats. params: host copy of the scorer's
+    // parameters (copied to the device). Needs allocate(); applies from the next launch.
+    void set_scorer(long itree, const float *params, long nparams);
+    // the (B, ntiles_dm, ntiles_t) logits of an acquired seq_id, and the tile counts
+    void output_scores(long seq_id, float *&scores, long &ntiles_dm, long &ntiles_t);
+
     // host: (nbatches_wt, B, ndm_wt, nt_wt, P, N) float32, all entries finite and >= 0.
     void fill_all_weights(long itree, const float *host, long nelts);
 
@@ -125,9 +134,9 @@ public:
     // Copies an internal buffer to the host (tests): name in {"grid", "lds<p>", "sb<t>"}.
     size_t debug_copy(const std::string &name, void *host, size_t maxbytes);
     // Peak-finder input rows of tree itree (the stage-2 output): row r = idm_out * M + m of global
-    // beam b, chunk sample t at base + (b * beam_stride + r * row_stride + t) * elt_bytes, valid
-    // for -tpad <= t < nt_ds (t < 0: the previous chunk's last samples). The rows hold the latest
-    // chunk whose stage 2 has run on the beam's batch.
+    // beam b, chunk sample t at base + (b * beam_stride + r * row_stride + t) * elt_bytes. After
+    // the chunk's peak finding (i.e. once its output is acquired) samples flmax <= t < nt_ds of
+    // the latest chunk of the beam's batch are there (flmax = 0 without subband final lags).
     void tree_rows(long itree, void *&base, long &nrows, long &beam_stride, long &row_stride, long &t_lo,
                    long &t_hi);
 
@@ -212,6 +221,9 @@ private:
     DeviceArray split_x;              // float32 scratch of the rank-8 split stage 2
     std::vector<size_t> out_offsets;                // [2 * itree + which]
     std::vector<DeviceArray> out_var; // [itree] (nbatches_out, B, ndm_out, M, P) float64, if enabled
+    // learned scorer (set_scorer): tree, parameters, tile origins of one batch, logits ring
+    long scorer_tree = -1, scorer_ntr = 0, scorer_ntt = 0;
+    DeviceArray scorer_params, scorer_origins, scorer_out;
     DeviceArray grid_rows;            // int (3*nchan) f0, cnt, woff
     DeviceArray grid_w;               // float
 

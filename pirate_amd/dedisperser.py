@@ -75,6 +75,7 @@ class GpuDedisperserOutputs:
         self.out_max = out_max
         self.out_argmax = out_argmax
         self.out_var = out_var      # [itree] float64 (B, ndm_out, M, nprofiles) if enable_variances
+        self.out_score = None       # float32 (B, rows / 64, samples / 256) learned-scorer logits (set_scorer)
         self.ichunk_zero_based = ichunk_zero_based
         self.ichunk_fpga_based = ichunk_fpga_based
         self.ibeam = ibeam
@@ -192,14 +193,26 @@ class GpuDedisperser:
 
     def tree_rows(self, itree):
         """scorer.TileSource over the peak-finder input rows of tree itree (its stage-2 output):
-        row idm_out * M + m of every global beam, sample t of the latest chunk whose stage 2 ran
-        on the beam's batch (t_lo = -history, t_hi = the tree's chunk samples). With one
-        full-band subband, row r is DM trial r. Valid after get_output of that chunk and
-        before the batch's next launch. Add tile origins with .with_origins()."""
+        row idm_out * M + m of every global beam, samples t_lo <= t < t_hi (t_hi = the tree's
+        chunk samples; t_lo = 0 without subband final lags) of the latest chunk of the beam's
+        batch. With one full-band subband, row r is DM trial r. Valid after get_output of that
+        chunk and before the batch's next launch. Add tile origins with .with_origins()."""
         from .scorer import TileSource
         s = L.TileSource()
         L.check(L.lib().pdd_dedisperser_tree_rows(self._h, itree, ctypes.byref(s)))
         return TileSource(s, [self])
+
+    def set_scorer(self, scorer, itree=0):
+        """Runs the learned scorer (a pirate_amd.LearnedScorer or its parameter vector) inside
+        the chain from the next launch on: every chunk's rows of tree itree are cut into tiles
+        of 64 DM trials x 256 samples on a fixed grid and scored right after the tree's stage 2;
+        the acquired outputs then carry out_score, a (B, rows / 64, samples / 256) float32
+        tensor of logits."""
+        p = scorer.params.detach().cpu().numpy() if hasattr(scorer, 'params') else np.asarray(scorer)
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        L.check(L.lib().pdd_dedisperser_set_scorer(self._h, itree, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                   p.size))
+        self._scorer_tree = itree
 
     def time(self, niterations, log=print):
         """Timing loop of the reference's GpuDedisperser::time (src_lib/GpuDedisperser.cpp:1573-1690):
@@ -238,8 +251,15 @@ class GpuDedisperser:
             ov = [device_view(vp[i], (self.beams_per_batch, t.ndm_out, t.nmultiplets_M, t.nprofiles), 'float64', owner=self)
                   for i, t in enumerate(self.trees)]
         ichunk = seq_id // self.nbatches
-        return GpuDedisperserOutputs(mx, am, ichunk, ichunk + self.initial_chunk,
+        outs = GpuDedisperserOutputs(mx, am, ichunk, ichunk + self.initial_chunk,
                                      (seq_id % self.nbatches) * self.beams_per_batch, ov)
+        if getattr(self, '_scorer_tree', None) is not None:
+            sp = ctypes.c_void_p()
+            ntr = ctypes.c_int64()
+            ntt = ctypes.c_int64()
+            L.check(L.lib().pdd_output_scores(self._h, seq_id, ctypes.byref(sp), ctypes.byref(ntr), ctypes.byref(ntt)))
+            outs.out_score = device_view(sp.value, (self.beams_per_batch, ntr.value, ntt.value), 'float32', owner=self)
+        return outs
 
     def _release_output(self, consumer_id, seq_id, stream_ptr):
         L.check(L.lib().pdd_release_output(self._h, consumer_id, seq_id, ctypes.c_void_p(stream_ptr)))

@@ -167,13 +167,43 @@ def test_tree_rows_of_a_dedisperser():
             pass
     src = dd.tree_rows(0)
     st = src.struct
-    assert st.nrows == 1024 and st.t_hi == 2048 and st.t_lo < 0 and st.nbeams == 2
-    origins = np.array([[0, 0, 0], [1, 960, 2048 - 256], [1, 64, -int(-st.t_lo)], [0, 512, 1000]], dtype=np.int32)
+    assert st.nrows == 1024 and st.t_hi == 2048 and st.t_lo == 0 and st.nbeams == 2
+    origins = np.array([[0, 0, 0], [1, 960, 2048 - 256], [1, 64, 5], [0, 512, 1000]], dtype=np.int32)
     got = scr.forward(src.with_origins(origins)).cpu().numpy()
     host = dd.debug_copy('sb0').view(np.float32)
-    lead = (host.size - st.nbeams * st.beam_stride)       # element offset of the rows in their buffer
-    rows = host[lead:lead + st.nbeams * st.beam_stride].reshape(st.nbeams, st.nrows, st.row_stride)
-    tpad = -st.t_lo
+    tpad = 32                                              # 2 * max_width of the tree
+    rowlen = st.row_stride
+    lead = host.size - st.nbeams * st.beam_stride          # element offset of the rows in their buffer
+    rows = host[lead:lead + st.nbeams * st.beam_stride].reshape(st.nbeams, st.nrows, rowlen)
     tiles = np.stack([rows[b, r:r + 64, tpad + t:tpad + t + 256] for b, r, t in origins])
     want = scr.forward(torch.from_numpy(np.ascontiguousarray(tiles)).cuda()).cpu().numpy()
     np.testing.assert_array_equal(got, want)
+
+
+def test_scorer_in_chain():
+    """GpuDedisperser.set_scorer: the chain scores every chunk's tiles of tree 0 right after its
+    stage 2; the logits equal LearnedScorer.forward on the same tiles of the tree rows."""
+    import torch
+    import pirate_amd as pa
+    from gpu_cases import CASES
+    d = dict(CASES['cfg1'], beams_per_gpu=4, beams_per_batch=2)
+    plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
+    dd = pa.GpuDedisperser(plan)
+    dd.allocate()
+    dd.fill_host_weights(0)
+    scr = pa.LearnedScorer(_params(71))
+    dd.set_scorer(scr, 0)
+    gen = torch.Generator(device='cuda')
+    gen.manual_seed(72)
+    grid = np.array([[b, 64 * i, 256 * j] for b in range(2) for i in range(16) for j in range(8)], dtype=np.int32)
+    for seq in range(6):
+        with dd.get_input(seq) as arr:
+            arr.normal_(generator=gen)
+        with dd.get_output(seq) as out:
+            assert tuple(out.out_score.shape) == (2, 16, 8)
+            got = out.out_score.cpu().numpy().reshape(-1)
+            src = dd.tree_rows(0)
+            src.struct.base = src.struct.base + out.ibeam * src.struct.beam_stride * 4
+            want = scr.forward(src.with_origins(grid)).cpu().numpy()
+        np.testing.assert_array_equal(got, want)
+        assert np.all(np.isfinite(got))
