@@ -10,15 +10,17 @@
 //                    the even row, then the "large" lag L_j = 2^(s-1) - 1 - (j mod 2^(s-1)))
 //
 // Mapping: one wave per (tree-channel pair (2j, 2j+1), time tile of 64 x G samples),
-// G = max(4, 2^(npri-1)); a workgroup holds 4 consecutive tiles of one pair. Waves never
-// wait on each other (no barriers; LDS is private to each wave). The wave grids its tile in
-// a coalesced layout (lane l: 4 consecutive samples in each 256-sample block, so every
-// 16-byte load and store instruction covers 1 KB of a row), stores the gridded rows (the
-// input of primary tree 0), and transposes them through LDS to G consecutive samples per
-// lane for the downsampling levels, so the gridded chunk is never read back from HBM. The
-// one-sample lag crosses lanes with a shuffle; lane 0 recomputes the previous group of the
-// even row itself (for the first tile: the previous chunk's last value, from per-beam
-// state). The large lag is a write offset: outputs past the chunk end go to per-beam state
+// G = max(4 (8 for 16-bit storage), 2^(npri-1)); a workgroup holds 4 consecutive
+// (pair, tile) waves in pair-major order (4 tiles of one pair, or 2 whole pairs when a pair
+// has 2 tiles). The wave grids its tile in a coalesced layout (lane l: 16 bytes of
+// consecutive samples in each block of 64 x 16 bytes, so every load and store instruction
+// covers 1 KB of a row), stores the gridded rows (the input of primary tree 0), and
+// transposes them through LDS (a per-wave buffer) to G consecutive samples per lane for the
+// downsampling levels, so the gridded chunk is never read back from HBM. The one-sample lag
+// crosses lanes with a shuffle; lane 0 takes the even row's value just before the tile from
+// the previous wave of the workgroup when that wave holds the previous tile (one barrier per
+// workgroup), else recomputes it from the input (for the first tile: the previous chunk's
+// last value, from per-beam state). The large lag is a write offset: outputs past the chunk end go to per-beam state
 // and are copied to the next chunk's first L_j samples. The state is double-buffered by
 // chunk parity, so the first tile's reads and the last tile's writes of one launch never
 // touch the same memory.
@@ -53,16 +55,13 @@ long lds_state_nelts(int R, int npri, int s)
     return 2 * (npri - 1) * nrows * (1 + lcap);     // two parities
 }
 
+// Samples per lane in the downsampling layout: at least 4, and 8 for 16-bit storage so that
+// the gridding loads and stores are 16 bytes per lane (the chunk length is a multiple of
+// 64 x 2^(npri-1) samples for 16-bit types, so 8 always divides it).
 template<typename Tst>
-__device__ __forceinline__ float quant(float x)
+__host__ __device__ constexpr int grid_lds_min_group()
 {
-    if constexpr (std::is_same<Tst, float>::value)
-        return x;
-    else {
-        Tst q;
-        st_st(&q, x);
-        return ld_st(&q);
-    }
+    return sizeof(Tst) == 2 ? 8 : 4;
 }
 
 // Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/V groups of V consecutive
@@ -119,24 +118,28 @@ template<typename Tst, int NPRI>
 __global__ void __launch_bounds__(kGlWaves * 64)
 k_grid_lds(GridLdsArgs a)
 {
-    constexpr int G = ((1 << (NPRI - 1)) > 4) ? (1 << (NPRI - 1)) : 4;
+    constexpr int G = std::max(grid_lds_min_group<Tst>(), 1 << (NPRI - 1));
     constexpr int NL = (NPRI > 1) ? NPRI : 2;          // array size for per-level values
 
     const int lane = threadIdx.x & 63;
     const int ngroups = a.ntime / G;
     const int ntiles = (ngroups + 63) / 64;
-    const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;          // workgroups per channel pair
     int blk = blockIdx.x;
     const int run = a.xcd_run;
     if (run > 0 && gridDim.x % (8 * run) == 0) {
         const int x = blk & 7, slot = blk >> 3;                 // workgroup blk runs on XCD blk mod 8
         blk = (slot / run) * (8 * run) + x * run + (slot % run);
     }
-    const int j = blk / tpb;                                       // channel pair
-    const int tile = (blk - j * tpb) * kGlWaves + (threadIdx.x >> 6);
+    // Waves are numbered pair-major over (pair, tile), so a workgroup holds 4 consecutive
+    // tiles of one pair, or several whole pairs when a pair has fewer than 4 tiles. The wave
+    // index is made wave-uniform (scalar), so that the pair's channel ranges, weights and
+    // loop bounds stay in scalar registers.
+    const int wv = blk * kGlWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = wv / ntiles;                                     // channel pair
+    const int tile = wv - j * ntiles;
     const int b = blockIdx.y;
     const int nrows = a.nchan >> 1;
-    if (tile >= ntiles)
+    if (j >= nrows)
         return;
 
     const int fe = a.row_f0[2 * j], ce = a.row_cnt[2 * j];
@@ -190,20 +193,27 @@ k_grid_lds(GridLdsArgs a)
         for (int p = 1; p < NPRI; p++)
             outp[p] = reinterpret_cast<Tst *>(a.out[p]) + long(b) * a.out_bstride[p] + long(j) * (a.ntime >> p);
 
-        // Lane 0: the even row's level-p value just before this tile (previous chunk's last
-        // value for the first tile, else recomputed from the previous group of samples).
+        // Lane 0: the even row's level-p value just before this tile. For the first tile it
+        // is the previous chunk's last value (per-beam state). When the previous tile is the
+        // previous wave of this workgroup (every wave but the first, in a workgroup whose 4
+        // waves all have work), that wave's last lane computes it from its own samples and
+        // hands it over through LDS, after one workgroup barrier (below). Otherwise lane 0
+        // recomputes it from the previous group of input samples.
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const bool full_wg = (long(blk) + 1) * kGlWaves <= long(nrows) * ntiles;
+        const bool from_prev_wave = full_wg && tile > 0 && w > 0;
         float prevs[NL];
         if (lane == 0) {
             if (tile == 0) {
 #pragma unroll
                 for (int p = 1; p < NPRI; p++)
                     prevs[p] = rd_small[long(p - 1) * nrows + j];
-            } else {
+            } else if (!from_prev_wave) {
                 float xp[G], xq[G];
                 grid_pair<Tst, G, 4, 4>(in + t0 - G, a.ntime, G / 4, fe, ce, we, fe, 0, we, xp, xq);
 #pragma unroll
                 for (int i = 0; i < G; i++)
-                    xp[i] = quant<Tst>(xp[i]);
+                    xp[i] = round_st<Tst>(xp[i]);
 #pragma unroll
                 for (int p = 1; p < NPRI; p++) {
                     const int n = G >> p;
@@ -253,8 +263,35 @@ k_grid_lds(GridLdsArgs a)
         }
 #pragma unroll
         for (int i = 0; i < G; i++) {
-            xe[i] = quant<Tst>(xe[i]);
-            xo[i] = quant<Tst>(xo[i]);
+            xe[i] = round_st<Tst>(xe[i]);
+            xo[i] = round_st<Tst>(xo[i]);
+        }
+        // Hand-over of the even row's last value per level to the next wave (the tile's last
+        // lane holds its last G samples, full tiles only: a tile followed by another is full).
+        // The full_wg condition is uniform over the workgroup, so every wave reaches the barrier.
+        __shared__ float tail[kGlWaves][NL];
+        if (full_wg) {
+            if (lane == 63) {
+                float xp[G];
+#pragma unroll
+                for (int i = 0; i < G; i++)
+                    xp[i] = xe[i];
+#pragma unroll
+                for (int p = 1; p < NPRI; p++) {
+                    const int n = G >> p;
+#pragma unroll
+                    for (int k = 0; k < (G >> 1); k++)
+                        if (k < n)
+                            xp[k] = kRsqrt2 * (xp[2 * k] + xp[2 * k + 1]);
+                    tail[w][p] = xp[n - 1];
+                }
+            }
+            __syncthreads();
+            if (from_prev_wave && lane == 0) {
+#pragma unroll
+                for (int p = 1; p < NPRI; p++)
+                    prevs[p] = tail[w - 1][p];
+            }
         }
         const bool last_tile = (tile == ntiles - 1);
 #pragma unroll
@@ -305,10 +342,9 @@ k_grid_lds(GridLdsArgs a)
 template<typename Tst>
 static void launch_grid_lds_t(const GridLdsArgs &a, hipStream_t st)
 {
-    const int G = std::max(4, 1 << (a.npri - 1));
-    const int ntiles = (a.ntime / G + 63) / 64;
-    const int tpb = (ntiles + kGlWaves - 1) / kGlWaves;
-    dim3 grid((a.nchan >> 1) * tpb, a.B);      // workgroups of 4 independent (pair, tile) waves
+    const int G = std::max(grid_lds_min_group<Tst>(), 1 << (a.npri - 1));
+    const long nwaves = long(a.nchan >> 1) * ((a.ntime / G + 63) / 64);
+    dim3 grid(unsigned((nwaves + kGlWaves - 1) / kGlWaves), a.B);   // 4 independent (pair, tile) waves each
     dim3 block(kGlWaves * 64);
     switch (a.npri) {
         case 1: hipLaunchKernelGGL((k_grid_lds<Tst, 1>), grid, block, 0, st, a); break;
