@@ -4,11 +4,11 @@
 // 16-bit types are storage only (converted with round-to-nearest-even on store).
 //
 // The st*_st helpers store to global memory with the nontemporal hint: every array they
-// write (gridded rows, downsampled rows, lag rings, peak-finder rows, outputs) is streamed
-// once per chunk and far larger than the caches, so keeping its lines in L2 only evicts
-// lines that are still to be read. Measured on MI355X (chord16k float32 chain): gridding
-// 1.16 -> 1.08 ms, stage2_t0 1.04 -> 1.01 ms, step 5.33 -> 5.20 ms. The same hint on the
-// loads is slower (stage2_t0 1.12 ms): neighbouring tiles of a ring row share lines.
+// write (gridded rows, lag rings, peak-finder rows, outputs) is streamed once per chunk and
+// far larger than the caches, so keeping its lines in L2 only evicts lines that are still to
+// be read. Measured on MI355X (chord16k float32 chain): gridding 1.16 -> 1.10 ms, stage2_t0
+// 1.04 -> 1.01 ms, step 5.33 -> 5.20 ms. The same hint on the loads is slower (stage2_t0
+// 1.12 ms): neighbouring tiles of a ring row share lines. st_st_cached is the plain store.
 // Every kernel is compiled with -ffp-contract=off: each multiply and add rounds
 // separately, which makes the float32 path bit-identical to the CPU oracle.
 
@@ -62,6 +62,14 @@ __device__ __forceinline__ float round_st(float x)
         return ld_st(&q);
     }
 }
+
+// One element to global memory, without the nontemporal hint: for rows whose stores start at
+// unaligned offsets, so that a line that one wave writes in part stays in L2 until the wave
+// holding the rest of it writes it (the lagged-downsampling outputs: 1.10 -> 1.08 ms for the
+// chord16k gridding kernel).
+__device__ __forceinline__ void st_st_cached(float *p, float x) { *p = x; }
+__device__ __forceinline__ void st_st_cached(__half *p, float x) { *p = __float2half(as_f32(x)); }
+__device__ __forceinline__ void st_st_cached(__hip_bfloat16 *p, float x) { *p = __float2bfloat16(x); }
 
 // One element to global memory (nontemporal).
 __device__ __forceinline__ void st_st(float *p, float x) { __builtin_nontemporal_store(x, p); }

@@ -139,7 +139,9 @@ def main(argv=None):
             for line in f:
                 line = line.strip()
                 if line.startswith('{'):
-                    alg = {k: v['alg_bytes'] for k, v in json.loads(line)['config']['per_kernel'].items()}
+                    cfg = json.loads(line)['config']
+                    pk = cfg.get('per_kernel_serialized', cfg.get('per_kernel', {}))
+                    alg = {k: v['alg_bytes'] for k, v in pk.items()}
     fetch = write = None
     if a.fetch:
         fscale, fcal, fetch = counter_bytes(a.fetch, 'FETCH_SIZE', names)
