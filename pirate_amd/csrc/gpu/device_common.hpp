@@ -19,6 +19,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 namespace pirate_amd {
 
 static constexpr float kRsqrt2 = 0.7071067811865476f;
@@ -34,6 +36,24 @@ __device__ __forceinline__ long launch_chunk(const A &a)
 __device__ __forceinline__ float ld_st(const float *p) { return *p; }
 __device__ __forceinline__ float ld_st(const __half *p) { return __half2float(*p); }
 __device__ __forceinline__ float ld_st(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
+
+// A storage element as loaded, before conversion: the float itself, or the 16 storage bits
+// zero-extended to 32. Kernels that hold loaded rows across other work keep them in this form
+// and convert at first use, so that each load writes the register that holds the row and the
+// compiler needs no temporary per load: with the conversion right at the load, the 16-bit
+// rank-7 stage 2 reused one temporary for all of its rows and waited for each load before
+// issuing the next.
+template<typename Tst>
+using raw_st = typename std::conditional<sizeof(Tst) == 4, float, unsigned>::type;
+
+__device__ __forceinline__ float ld_raw(const float *p) { return *p; }
+__device__ __forceinline__ unsigned ld_raw(const __half *p) { return *reinterpret_cast<const unsigned short *>(p); }
+__device__ __forceinline__ unsigned ld_raw(const __hip_bfloat16 *p) { return *reinterpret_cast<const unsigned short *>(p); }
+
+template<typename Tst> __device__ __forceinline__ float from_raw(raw_st<Tst> v);
+template<> __device__ __forceinline__ float from_raw<float>(float v) { return v; }
+template<> __device__ __forceinline__ float from_raw<__half>(unsigned v) { return __half2float(__ushort_as_half((unsigned short) v)); }
+template<> __device__ __forceinline__ float from_raw<__hip_bfloat16>(unsigned v) { return __uint_as_float(v << 16); }
 
 // The value as computed in float32. Without this the compiler may fuse the multiply that
 // produced x into the float16 conversion (v_fma_mix*_f16), rounding once from the exact

@@ -41,9 +41,10 @@ using namespace treg;
 constexpr int cmin(int a, int b) { return (a < b) ? a : b; }
 
 // Phase-A rows (of 16) of the next tile that rank-7 stage 2 loads during the current tile's
-// phase B. Measured on chord16k (stage2_t0, fp32 / fp16 ms): 0 rows 1.20 / 1.61, 4 rows
-// 1.07 / 1.46, 8 rows 1.07 / 1.52, 12 rows 1.14 / 1.60 (spills in the tile loop), 16 rows 1.31.
-constexpr int kStage2Rank7Prefetch = 4;
+// phase B. Measured on chord16k (stage2_t0, fp32 / fp16 ms): 0 rows 1.10 / 0.87, 4 rows
+// 1.01 / 0.88, 8 rows 0.99 / 0.86; 12 rows spilled in the tile loop (fp32 1.14 vs 1.07 for 8
+// in an earlier build), all 16 rows 1.31.
+constexpr int kStage2Rank7Prefetch = 8;
 
 template<int DD>
 struct FastDims {
@@ -314,7 +315,12 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
 
     // phase-A input rows of tile t0 (issued one tile ahead); PART: a half tile, whose lanes
     // past the chunk end read zeros
-    float xa[F::SA][F::NRA];
+    // Stage 2 keeps them as loaded (raw_st) and converts where phase A uses them: converted
+    // at the load, its 16-bit variant waited for each of a tile's row loads before issuing the
+    // next (one temporary register for all). Stage 1 converts at the load: kept raw, the
+    // compiler packs two rows per register and waits for each pair instead.
+    using LdT = typename std::conditional<STAGE2, raw_st<Tst>, float>::type;
+    LdT xa[F::SA][F::NRA];
     // Stage 1 (and stage 2 below rank 7) issues all of the next tile's phase-A loads right
     // after this tile's phase A, so they are in flight during phase B; rank-7 stage 2 would
     // spill with all of them (1.31 vs 1.19 ms on chord16k) and prefetches the first NPF rows.
@@ -334,7 +340,7 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
                 } else {
                     const int4 d = idesc[h * F::NRA + l];
                     const int tl = t0 + lane;
-                    xa[s][l] = live ? ld_st(src + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : 0.0f;
+                    xa[s][l] = live ? ld_raw(src + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : LdT(0);
                 }
             }
         }
@@ -355,8 +361,16 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
             load_a(t0, half, RP{}, RN{});
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
-            sub_passes<F::K1>(xa[s], ha[s], lane);
-            phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
+            float x[F::NRA];
+#pragma unroll
+            for (int l = 0; l < F::NRA; l++) {
+                if constexpr (STAGE2)
+                    x[l] = from_raw<Tst>(xa[s][l]);
+                else
+                    x[l] = xa[s][l];
+            }
+            sub_passes<F::K1>(x, ha[s], lane);
+            phase_a_write<DD>(lbuf, wid + s * F::NW, x, lane);
             if constexpr (F::SA > 1 && DD >= 8)
                 __builtin_amdgcn_sched_barrier(0);    // one sub-tree at a time (register pressure)
         }
