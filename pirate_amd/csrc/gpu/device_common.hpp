@@ -106,26 +106,6 @@ typedef float f32x4_nt __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2_nt __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4_nt __attribute__((ext_vector_type(4)));
 
-// Four consecutive storage elements (16-byte aligned for float, 8-byte for 16-bit types).
-__device__ __forceinline__ float4 ld4_st(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-
-__device__ __forceinline__ float4 ld4_st(const __half *p)
-{
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    return make_float4(__half2float(__ushort_as_half((unsigned short) (u.x & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short) (u.x >> 16))),
-                       __half2float(__ushort_as_half((unsigned short) (u.y & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short) (u.y >> 16))));
-}
-
-__device__ __forceinline__ float4 ld4_st(const __hip_bfloat16 *p)
-{
-    // bfloat16 -> float is exact: the 16 bits are the high half of the float.
-    const uint2 u = *reinterpret_cast<const uint2 *>(p);
-    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
-                       __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
-}
-
 // Four consecutive storage elements from float (round-to-nearest-even for 16-bit types),
 // to global memory (nontemporal).
 __device__ __forceinline__ void st4_st(float *p, float4 v)
@@ -146,25 +126,36 @@ __device__ __forceinline__ void st4_st(__half *p, float4 v) { st4_st16(p, v); }
 __device__ __forceinline__ void st4_st(__hip_bfloat16 *p, float4 v) { st4_st16(p, v); }
 
 // V consecutive storage elements <-> floats: V = 4 (16-byte float or 8-byte 16-bit access) or,
-// for 16-bit types, V = 8 (one 16-byte access). Stores are nontemporal.
+// for 16-bit types, V = 8 (one 16-byte access). Stores are nontemporal. Loads come in two
+// steps, as for raw_st: ldv_raw (the bits as loaded) and from_rawv (to floats, at first use).
 template<typename Tst, int V>
-__device__ __forceinline__ void ldv_st(const Tst *p, float *x)
+using rawv_st = typename std::conditional<sizeof(Tst) == 4, float4,
+                                          typename std::conditional<V == 8, uint4, uint2>::type>::type;
+
+template<typename Tst, int V>
+__device__ __forceinline__ rawv_st<Tst, V> ldv_raw(const Tst *p)
 {
-    if constexpr (V == 4) {
-        const float4 v = ld4_st(p);
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    static_assert(V == 4 || (V == 8 && sizeof(Tst) == 2), "4-wide access, or 8-wide for 16-bit types");
+    return *reinterpret_cast<const rawv_st<Tst, V> *>(p);
+}
+
+template<typename Tst, int V>
+__device__ __forceinline__ void from_rawv(const rawv_st<Tst, V> &r, float *x)
+{
+    if constexpr (sizeof(Tst) == 4) {
+        x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w;
     } else {
-        static_assert(V == 8 && sizeof(Tst) == 2, "8-wide access is for 16-bit types");
-        const uint4 u = *reinterpret_cast<const uint4 *>(p);
-        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+        unsigned w[V / 2];
+        w[0] = r.x;
+        w[1] = r.y;
+        if constexpr (V == 8) {
+            w[2] = r.z;
+            w[3] = r.w;
+        }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            Tst lo, hi;
-            const unsigned short l16 = (unsigned short) (w[k] & 0xffffu), h16 = (unsigned short) (w[k] >> 16);
-            __builtin_memcpy(&lo, &l16, 2);
-            __builtin_memcpy(&hi, &h16, 2);
-            x[2 * k] = ld_st(&lo);
-            x[2 * k + 1] = ld_st(&hi);
+        for (int k = 0; k < V / 2; k++) {
+            x[2 * k] = from_raw<Tst>(w[k] & 0xffffu);
+            x[2 * k + 1] = from_raw<Tst>(w[k] >> 16);
         }
     }
 }

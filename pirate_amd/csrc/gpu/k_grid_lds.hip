@@ -81,37 +81,36 @@ __device__ __forceinline__ void grid_pair(const Tst *in, long ntime, int nq, int
         xe[i] = xo[i] = 0.0f;
     const int fa = min(fe, fo), fb = max(fe + ce, fo + co);
     for (int f1 = fa; f1 < fb; f1 += NB) {
-        float v[NB][G];
+        // as loaded, converted where used (16-bit types: each load keeps its own registers,
+        // so a batch's loads are all in flight at once)
+        rawv_st<Tst, V> v[NB][G / V];
 #pragma unroll
         for (int u = 0; u < NB; u++) {
             const Tst *p = in + long(min(f1 + u, fb - 1)) * ntime;
 #pragma unroll
-            for (int q = 0; q < G / V; q++) {
-                if (q < nq) {
-                    ldv_st<Tst, V>(p + QS * q, &v[u][V * q]);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < V; c++)
-                        v[u][V * q + c] = 0.0f;
-                }
-            }
+            for (int q = 0; q < G / V; q++)
+                v[u][q] = (q < nq) ? ldv_raw<Tst, V>(p + QS * q) : rawv_st<Tst, V>{};
         }
 #pragma unroll
         for (int u = 0; u < NB; u++) {
             const int f = f1 + u;
             if (f >= fb)
                 break;
+            float x[G];
+#pragma unroll
+            for (int q = 0; q < G / V; q++)
+                from_rawv<Tst, V>(v[u][q], &x[V * q]);
             if (f >= fe && f < fe + ce) {
                 const float wk = we[f - fe];
 #pragma unroll
                 for (int i = 0; i < G; i++)
-                    xe[i] = xe[i] + wk * v[u][i];
+                    xe[i] = xe[i] + wk * x[i];
             }
             if (f >= fo && f < fo + co) {
                 const float wk = wo[f - fo];
 #pragma unroll
                 for (int i = 0; i < G; i++)
-                    xo[i] = xo[i] + wk * v[u][i];
+                    xo[i] = xo[i] + wk * x[i];
             }
         }
     }

@@ -287,21 +287,23 @@ k_pf(PfArgs a)
     for (int it = 0; it < ntiles; it++) {
         const int p0 = a.tpad + (tile0 + it - nwarm) * TL;
 
-        // ---- stage [nstreams][TL] samples (4 per load, VPR loads per row segment)
+        // ---- stage [nstreams][TL] samples (4 per load, VPR loads per row segment); a thread
+        // issues all of its loads before it converts and stores them (with the conversion at
+        // each load, the 16-bit variants waited for each load before issuing the next)
+        rawv_st<Tst, 4> raw[VPR];
 #pragma unroll
         for (int k = 0; k < VPR; k++) {
             const int idx = tid + k * kPfThreads;
             const int s = idx / VPR, i = (idx - s * VPR) * 4;
-            if (s < nstreams) {
-                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (p0 + i >= 0)
-                    v = ld4_st(rows + long(s) * a.rowlen + p0 + i);
-                float *d = tile + s * STR + i;
-                d[0] = v.x;
-                d[1] = v.y;
-                d[2] = v.z;
-                d[3] = v.w;
-            }
+            raw[k] = (s < nstreams && p0 + i >= 0) ? ldv_raw<Tst, 4>(rows + long(s) * a.rowlen + p0 + i)
+                                                  : rawv_st<Tst, 4>{};
+        }
+#pragma unroll
+        for (int k = 0; k < VPR; k++) {
+            const int idx = tid + k * kPfThreads;
+            const int s = idx / VPR, i = (idx - s * VPR) * 4;
+            if (s < nstreams)
+                from_rawv<Tst, 4>(raw[k], tile + s * STR + i);
         }
         __syncthreads();
 
