@@ -175,6 +175,14 @@ __device__ __forceinline__ void stv_st(Tst *p, const float *x)
     }
 }
 
+// Read-only table entry through the constant address space: with a wave-uniform index the
+// compiler issues a scalar load (scalar cache) instead of a vector memory round trip. For
+// tables that no kernel writes while it runs.
+__device__ __forceinline__ int ld_table(const int *p, int i)
+{
+    return ((const __attribute__((address_space(4))) int *) p)[i];
+}
+
 __device__ __forceinline__ int bitrev_dev(int i, int nbits)
 {
     return (nbits > 0) ? int(__brev(unsigned(i)) >> (32 - nbits)) : 0;

@@ -81,7 +81,9 @@ __device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const
     for (int i = 0; i < TD::ITEMS; i++) {
         const int idx = i * kTreeThreads + tid;
         if (idx < TD::WORK) {
-            const int p = idx / kTT, t = idx % kTT;
+            // p is uniform over a wave (kTT = 64 samples per pair): scalar, so that the
+            // emission-table and final-lag reads are scalar loads
+            const int p = __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
             const int dmb = p & (half - 1);
             const int e = ((p >> r) << (r + 1)) + dmb;
             const int o = e + half;
@@ -90,7 +92,7 @@ __device__ __forceinline__ void tree_pass(int r, float *tile, float *hist, const
             if (t >= kTT - half)
                 h[p * half + (t - (kTT - half))] = sv[i];
             if (emit) {
-                const int s0 = emit[r * TD::NP + p];
+                const int s0 = ld_table(emit, r * TD::NP + p);
                 if (s0 >= 0) {
                     sink.emit(s0, t, o0[i]);
                     sink.emit(s0 + 1, t, o1[i]);
@@ -152,7 +154,7 @@ struct Stage2Sink {
             return;
         const int pf_dm = s / M, m = s - pf_dm * M;
         const long row = long(dpf_base + pf_dm) * M + m;
-        st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
+        st_st(sbb + row * rowlen + tpad + t0 + t + ld_table(fl, s), v);
     }
 
     __device__ void read_phase_extra(int) { }
@@ -161,17 +163,17 @@ struct Stage2Sink {
     __device__ void stag_rounds(int r)
     {
         const int half = 1 << r;
-        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+        const int k0 = ld_table(stag_begin, r), n = (ld_table(stag_begin, r + 1) - k0) * kTT;
         for (int base = 0; base < n; base += SMAX * kTreeThreads) {
             float so0[SMAX], so1[SMAX], ssv[SMAX];
 #pragma unroll
             for (int q = 0; q < SMAX; q++) {
                 const int idx = base + q * kTreeThreads + threadIdx.x;
                 if (idx < n) {
-                    const int k = k0 + idx / kTT, t = idx % kTT;
-                    const int *e = stag + 6 * k;
-                    const int row0 = e[1], row1 = e[2], L = e[3];
-                    const float *h = shist + e[5];
+                    const int k = k0 + __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;   // k: wave-uniform
+                    const int row0 = ld_table(stag, 6 * k + 1), row1 = ld_table(stag, 6 * k + 2);
+                    const int L = ld_table(stag, 6 * k + 3);
+                    const float *h = shist + ld_table(stag, 6 * k + 5);
                     const float xo = tile[row1 * kTT + t];
                     const float xa = (t >= L) ? tile[row0 * kTT + t - L] : h[half + t - L];
                     const float xb = (t >= L + 1) ? tile[row0 * kTT + t - L - 1] : h[half + t - L - 1];
@@ -186,12 +188,12 @@ struct Stage2Sink {
             for (int q = 0; q < SMAX; q++) {
                 const int idx = base + q * kTreeThreads + threadIdx.x;
                 if (idx < n) {
-                    const int k = k0 + idx / kTT, t = idx % kTT;
-                    const int *e = stag + 6 * k;
-                    emit(e[4], t, so0[q]);
-                    emit(e[4] + 1, t, so1[q]);
+                    const int k = k0 + __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
+                    const int s0 = ld_table(stag, 6 * k + 4);
+                    emit(s0, t, so0[q]);
+                    emit(s0 + 1, t, so1[q]);
                     if (t >= kTT - half)
-                        shist[e[5] + (t - (kTT - half))] = ssv[q];
+                        shist[ld_table(stag, 6 * k + 5) + (t - (kTT - half))] = ssv[q];
                 }
             }
             __syncthreads();
@@ -289,12 +291,17 @@ k_stage2(Stage2Args a)
                 for (int gg = 0; gg < PP::GPW; gg++) {
                     const int grp = wid + gg * PP::NWV;
                     if (grp < PP::NG) {
-                        float v[SK::NR];
+                        // rows as loaded, converted after all of them are issued (raw_st)
+                        raw_st<Tst> vr[SK::NR];
 #pragma unroll
                         for (int l = 0; l < SK::NR; l++) {
                             const int4 d = desc[grp * SK::NR + l];
-                            v[l] = live ? ld_st(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : 0.0f;
+                            vr[l] = live ? ld_raw(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : raw_st<Tst>(0);
                         }
+                        float v[SK::NR];
+#pragma unroll
+                        for (int l = 0; l < SK::NR; l++)
+                            v[l] = from_raw<Tst>(vr[l]);
                         sub_passes<KP>(v, hreg[gg], lane);
 #pragma unroll
                         for (int l = 0; l < SK::NR; l++)
@@ -313,8 +320,8 @@ k_stage2(Stage2Args a)
             for (int r = KP; r < DD; r++) {
                 if (r == r_lvl0) {
                     for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
-                        const int k = idx / kTT, t = idx % kTT;
-                        sink.emit(lvl0_tab[2 * k + 1], t, tile[lvl0_tab[2 * k] * kTT + t]);
+                        const int k = __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
+                        sink.emit(ld_table(lvl0_tab, 2 * k + 1), t, tile[ld_table(lvl0_tab, 2 * k) * kTT + t]);
                     }
                 }
                 sink.stag_rounds(r);
@@ -322,8 +329,8 @@ k_stage2(Stage2Args a)
             }
             if (r_lvl0 == DD) {
                 for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
-                    const int k = idx / kTT, t = idx % kTT;
-                    sink.emit(lvl0_tab[2 * k + 1], t, tile[lvl0_tab[2 * k] * kTT + t]);
+                    const int k = __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
+                    sink.emit(ld_table(lvl0_tab, 2 * k + 1), t, tile[ld_table(lvl0_tab, 2 * k) * kTT + t]);
                 }
                 __syncthreads();
             }
@@ -426,7 +433,7 @@ struct SplitSink {
             return;
         const int pf_dm = s / M, m = s - pf_dm * M;
         const long row = long(dpf_base + pf_dm) * M + m;
-        st_st(sbb + row * rowlen + tpad + t0 + t + fl[s], v);
+        st_st(sbb + row * rowlen + tpad + t0 + t + ld_table(fl, s), v);
     }
 
     // Staggered butterflies of pass r whose rows are in this workgroup's groups, before the
@@ -434,20 +441,20 @@ struct SplitSink {
     __device__ void stag_rounds(int r)
     {
         const int half = 1 << r;
-        const int k0 = stag_begin[r], n = (stag_begin[r + 1] - k0) * kTT;
+        const int k0 = ld_table(stag_begin, r), n = (ld_table(stag_begin, r + 1) - k0) * kTT;
         for (int base = 0; base < n; base += SMAX * kTreeThreads) {
             float so0[SMAX], so1[SMAX], ssv[SMAX][HB];
 #pragma unroll
             for (int q = 0; q < SMAX; q++) {
                 const int idx = base + q * kTreeThreads + threadIdx.x;
                 if (idx < n) {
-                    const int k = k0 + idx / kTT, t = idx % kTT;
-                    const int *e = stag + 6 * k;
-                    const unsigned gi = unsigned((e[1] & 15) - g0);
+                    const int k = k0 + __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;   // k: wave-uniform
+                    const int e1 = ld_table(stag, 6 * k + 1);
+                    const unsigned gi = unsigned((e1 & 15) - g0);
                     if (gi < unsigned(kSplitGPB)) {
                         const float *tg = tile + gi * (16 * kTT);
-                        const int row0 = e[1] >> 4, row1 = e[2] >> 4, L = e[3];
-                        const float *h = stl + gi * nstg + kSplitHistB + stag_goff[k];
+                        const int row0 = e1 >> 4, row1 = ld_table(stag, 6 * k + 2) >> 4, L = ld_table(stag, 6 * k + 3);
+                        const float *h = stl + gi * nstg + kSplitHistB + ld_table(stag_goff, k);
                         const float xo = tg[row1 * kTT + t];
                         const float xa = (t >= L) ? tg[row0 * kTT + t - L] : h[half + t - L];
                         const float xb = (t >= L + 1) ? tg[row0 * kTT + t - L - 1] : h[half + t - L - 1];
@@ -470,13 +477,13 @@ struct SplitSink {
             for (int q = 0; q < SMAX; q++) {
                 const int idx = base + q * kTreeThreads + threadIdx.x;
                 if (idx < n) {
-                    const int k = k0 + idx / kTT, t = idx % kTT;
-                    const int *e = stag + 6 * k;
-                    const unsigned gi = unsigned((e[1] & 15) - g0);
+                    const int k = k0 + __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
+                    const unsigned gi = unsigned((ld_table(stag, 6 * k + 1) & 15) - g0);
                     if (gi < unsigned(kSplitGPB)) {
-                        emit(e[4], t, so0[q]);
-                        emit(e[4] + 1, t, so1[q]);
-                        float *h = stl + gi * nstg + kSplitHistB + stag_goff[k];
+                        const int s0 = ld_table(stag, 6 * k + 4);
+                        emit(s0, t, so0[q]);
+                        emit(s0 + 1, t, so1[q]);
+                        float *h = stl + gi * nstg + kSplitHistB + ld_table(stag_goff, k);
                         if (half <= kTT) {
                             if (t >= kTT - half)
                                 h[t - (kTT - half)] = ssv[q][0];
@@ -550,8 +557,8 @@ __device__ __forceinline__ void split_pass(int r, int g0, float *tile, float *st
                     hq[m * kTT + t] = sv[i][m];
         }
         const int dmb = e & (half - 1);
-        const int p = ((e >> (r + 1)) << r) + dmb;
-        const int s0 = emit[r * 128 + p];
+        const int p = __builtin_amdgcn_readfirstlane(((e >> (r + 1)) << r) + dmb);   // wave-uniform (q, gi are)
+        const int s0 = ld_table(emit, r * 128 + p);
         if (s0 >= 0) {
             sink.emit(s0, t, o0[i]);
             sink.emit(s0 + 1, t, o1[i]);
@@ -600,11 +607,11 @@ k_stage2_split_b(Stage2Args a, const float *X)
     const int *lvl0_tab = a.lvl0;
     auto lvl0 = [&sink, tile, tid, g0, nlvl0, lvl0_tab]() {
         for (int idx = tid; idx < nlvl0 * kTT; idx += kTreeThreads) {
-            const int k = idx / kTT, t = idx % kTT;
-            const int row = lvl0_tab[2 * k];
+            const int k = __builtin_amdgcn_readfirstlane(idx / kTT), t = idx % kTT;
+            const int row = ld_table(lvl0_tab, 2 * k);
             const unsigned gi = unsigned((row & 15) - g0);
             if (gi < unsigned(GPB))
-                sink.emit(lvl0_tab[2 * k + 1], t, tile[gi * (16 * kTT) + (row >> 4) * kTT + t]);
+                sink.emit(ld_table(lvl0_tab, 2 * k + 1), t, tile[gi * (16 * kTT) + (row >> 4) * kTT + t]);
         }
     };
 
