@@ -291,17 +291,20 @@ k_stage2(Stage2Args a)
                 for (int gg = 0; gg < PP::GPW; gg++) {
                     const int grp = wid + gg * PP::NWV;
                     if (grp < PP::NG) {
-                        // rows as loaded, converted after all of them are issued (raw_st)
+                        // rows as loaded, converted after all of them are issued (raw_st);
+                        // lanes past the chunk end load the last sample (in bounds) and take
+                        // zero, so that no load sits under a per-lane branch
+                        const int tlc = live ? tl : Tp - 1;
                         raw_st<Tst> vr[SK::NR];
 #pragma unroll
                         for (int l = 0; l < SK::NR; l++) {
                             const int4 d = desc[grp * SK::NR + l];
-                            vr[l] = live ? ld_raw(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0))) : raw_st<Tst>(0);
+                            vr[l] = ld_raw(ring + (d.x + tlc - ((tlc >= d.y) ? d.z : 0)));
                         }
                         float v[SK::NR];
 #pragma unroll
                         for (int l = 0; l < SK::NR; l++)
-                            v[l] = from_raw<Tst>(vr[l]);
+                            v[l] = live ? from_raw<Tst>(vr[l]) : 0.0f;
                         sub_passes<KP>(v, hreg[gg], lane);
 #pragma unroll
                         for (int l = 0; l < SK::NR; l++)
@@ -619,9 +622,20 @@ k_stage2_split_b(Stage2Args a, const float *X)
         const int nv = min(kTT, a.Tp - t0);
         sink.t0 = t0;
         sink.nv = nv;
-        for (int rw = __builtin_amdgcn_readfirstlane(tid >> 6); rw < GPB * 16; rw += kTreeThreads / 64) {
-            const int gi = rw >> 4, l = rw & 15;
-            tile[rw * kTT + lane] = (lane < nv) ? x[long((l << 4) + g0 + gi) * a.Tp + t0 + lane] : 0.0f;
+        {
+            // each wave loads its RPW rows, then writes them to LDS (all loads in flight at once)
+            constexpr int NWV = kTreeThreads / 64, RPW = GPB * 16 / NWV;
+            static_assert(GPB * 16 % NWV == 0, "rows per wave");
+            const int w0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+            float xv[RPW];
+#pragma unroll
+            for (int k = 0; k < RPW; k++) {
+                const int rw = w0 + k * NWV, gi = rw >> 4, l = rw & 15;
+                xv[k] = (lane < nv) ? x[long((l << 4) + g0 + gi) * a.Tp + t0 + lane] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < RPW; k++)
+                tile[(w0 + k * NWV) * kTT + lane] = xv[k];
         }
         __syncthreads();
 #pragma unroll 1

@@ -554,10 +554,18 @@ k_stage2_split_a_reg(Stage2Args a, float *X, long split_nst)
     for (int t0 = 0; t0 < a.Tp; t0 += 64) {
         const int tl = t0 + lane;
         const bool live = tl < a.Tp;
+        // Lanes past the chunk end load the last sample instead (in bounds) and take zero:
+        // loads under a per-lane condition become branches, and with 16-bit storage the
+        // conversion after each branch waited for that load before the next was issued.
+        const int tlc = live ? tl : a.Tp - 1;
+        raw_st<Tst> vr[16];      // as loaded, converted after all 16 loads are issued
+#pragma unroll
+        for (int l = 0; l < 16; l++)
+            vr[l] = ld_raw(ring + (rbase[l] + tlc - ((tlc >= rwrap[l]) ? rcap[l] : 0)));
         float v[16];
 #pragma unroll
         for (int l = 0; l < 16; l++)
-            v[l] = live ? ld_st(ring + (rbase[l] + tl - ((tl >= rwrap[l]) ? rcap[l] : 0))) : 0.0f;
+            v[l] = live ? from_raw<Tst>(vr[l]) : 0.0f;
         sub_passes<4>(v, hh, lane);
         if (live) {
 #pragma unroll
