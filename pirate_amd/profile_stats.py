@@ -1,12 +1,13 @@
 """Markdown summary of a rocprofv3 --kernel-trace --stats run (its kernel_stats.csv).
 
     python -m pirate_amd.profile_stats DIR --md OUT.md --title TITLE --command CMD [--command CMD ...]
-                                      [--note TEXT ...] [--flop NAME_SUBSTRING=FLOP_PER_CALL ...]
+                                      [--note TEXT ...] [--flop NAME_SUBSTRING=FLOP_PER_CALL ...] [--lib-only]
 
 DIR is a rocprofv3 output directory written with ``--output-format csv -o run``. The table
 lists every kernel (name, calls, average / total time, share of the total); with --flop, the
 kernels whose name contains the substring also get an achieved TFLOP/s column (flop per call /
-average duration).
+average duration). --lib-only keeps this library's kernels (pirate_amd::) and drops those of
+torch and its libraries (test-data generation), whose share is then left out of the percentages.
 """
 
 import argparse
@@ -38,12 +39,18 @@ def main(argv=None):
     ap.add_argument('--command', action='append', default=[])
     ap.add_argument('--note', action='append', default=[])
     ap.add_argument('--flop', action='append', default=[])
+    ap.add_argument('--lib-only', action='store_true')
     a = ap.parse_args(argv)
     flop = {}
     for f in a.flop:
         k, v = f.split('=')
         flop[k] = float(v)
     rows = read_stats(a.dir)
+    if a.lib_only:
+        rows = [r for r in rows if 'pirate_amd::' in r['name']]
+        tot = sum(r['total_ns'] for r in rows) or 1.0
+        for r in rows:
+            r['pct'] = 100.0 * r['total_ns'] / tot
     lines = [f'# {a.title}', '', 'Commands (run from the repository root on the GPU host):', '']
     lines += ['    ' + c for c in a.command] + ['']
     for n in a.note:
