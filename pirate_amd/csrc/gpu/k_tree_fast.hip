@@ -43,8 +43,10 @@ constexpr int cmin(int a, int b) { return (a < b) ? a : b; }
 // Phase-A rows (of 16) of the next tile that rank-7 stage 2 loads during the current tile's
 // phase B. Measured on chord16k (stage2_t0, fp32 / fp16 ms): 0 rows 1.10 / 0.87, 4 rows
 // 1.01 / 0.88, 8 rows 0.99 / 0.86; 12 rows spilled in the tile loop (fp32 1.14 vs 1.07 for 8
-// in an earlier build), all 16 rows 1.31.
+// in an earlier build), all 16 rows 1.31. Rank 8 (which spills already) keeps 4: on a
+// chime.yml-shaped workload 0, 2, 4, 8 rows measured 3.70, 3.88, 3.70, 3.73 ms.
 constexpr int kStage2Rank7Prefetch = 8;
+constexpr int kStage2Rank8Prefetch = 4;
 
 template<int DD>
 struct FastDims {
@@ -324,7 +326,7 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
     // Stage 1 (and stage 2 below rank 7) issues all of the next tile's phase-A loads right
     // after this tile's phase A, so they are in flight during phase B; rank-7 stage 2 would
     // spill with all of them (1.31 vs 1.19 ms on chord16k) and prefetches the first NPF rows.
-    constexpr int NPF = (!STAGE2 || DD <= 6) ? F::NRA : kStage2Rank7Prefetch;
+    constexpr int NPF = (!STAGE2 || DD <= 6) ? F::NRA : (DD == 7) ? kStage2Rank7Prefetch : kStage2Rank8Prefetch;
     // rows [L0, L1) of tile t0
     auto load_a = [&](int t0, auto part, auto l0, auto l1) {
         constexpr bool PART = decltype(part)::value;
@@ -361,16 +363,17 @@ __device__ __forceinline__ void fast_tree_body(const Args &a)
             load_a(t0, half, RP{}, RN{});
 #pragma unroll
         for (int s = 0; s < F::SA; s++) {
-            float x[F::NRA];
+            if constexpr (STAGE2) {
+                float x[F::NRA];
 #pragma unroll
-            for (int l = 0; l < F::NRA; l++) {
-                if constexpr (STAGE2)
+                for (int l = 0; l < F::NRA; l++)
                     x[l] = from_raw<Tst>(xa[s][l]);
-                else
-                    x[l] = xa[s][l];
+                sub_passes<F::K1>(x, ha[s], lane);
+                phase_a_write<DD>(lbuf, wid + s * F::NW, x, lane);
+            } else {
+                sub_passes<F::K1>(xa[s], ha[s], lane);
+                phase_a_write<DD>(lbuf, wid + s * F::NW, xa[s], lane);
             }
-            sub_passes<F::K1>(x, ha[s], lane);
-            phase_a_write<DD>(lbuf, wid + s * F::NW, x, lane);
             if constexpr (F::SA > 1 && DD >= 8)
                 __builtin_amdgcn_sched_barrier(0);    // one sub-tree at a time (register pressure)
         }
