@@ -11,9 +11,9 @@ from .group import GpuDedisperserGroup
 from .kernels import GpuDequantizationKernel, GpuRingbufCopyKernel
 from .pf_variance import PfAvarApproximation
 from .offline import AssembledFrame, OfflineDedisperser, XEngineMetadata
-from .scorer import LearnedScorer, TileSource
+from .scorer import LearnedScorer, ScoreHead, TileSource
 
 __all__ = ['DedispersionConfig', 'DedispersionPlan', 'DedispersionTree', 'PrimaryTree',
            'GpuDedisperser', 'GpuDedisperserOutputs', 'GpuDedisperserGroup', 'OutputRingView', 'GpuDequantizationKernel', 'GpuRingbufCopyKernel',
            'PfAvarApproximation', 'OfflineDedisperser', 'AssembledFrame', 'XEngineMetadata', 'LearnedScorer',
-           'TileSource']
+           'ScoreHead', 'TileSource']

@@ -172,3 +172,58 @@ class LearnedScorer:
         _, _, loss = self.grad(src, labels, stream)
         self.adam_step(lr=lr, stream=stream)
         return loss.mean()
+
+
+class ScoreHead:
+    """Calibrated head over the network's logit z and the peak finder's boxcar statistic p of
+    the same tile: score = w . f(z, p), f = (1, z, p, z p), standardized on the training set,
+    with w the logistic-regression fit of the training labels (Newton / iteratively reweighted
+    least squares in float64, a small ridge term for conditioning). The network ranks tiles
+    by shape (it learns to reject RFI); the boxcar statistic carries the matched-filter SNR,
+    near optimal on Gaussian noise; the head weighs both. score is a log-odds."""
+
+    NF = 4
+
+    def __init__(self, w=None, mean=None, std=None):
+        self.w = None if w is None else np.asarray(w, dtype=np.float64)
+        self.mean = None if mean is None else np.asarray(mean, dtype=np.float64)
+        self.std = None if std is None else np.asarray(std, dtype=np.float64)
+
+    @staticmethod
+    def _raw(z, p):
+        z = np.asarray(z, dtype=np.float64)
+        p = np.asarray(p, dtype=np.float64)
+        return np.stack([z, p, z * p], axis=1)
+
+    def _features(self, z, p):
+        f = (self._raw(z, p) - self.mean) / self.std
+        return np.concatenate([np.ones((f.shape[0], 1)), f], axis=1)
+
+    def fit(self, z, p, labels, iters=30, ridge=1e-6):
+        """Fits w; returns the per-iteration mean log-loss."""
+        r = self._raw(z, p)
+        self.mean = r.mean(axis=0)
+        self.std = np.maximum(r.std(axis=0), 1e-12)
+        X = self._features(z, p)
+        y = np.asarray(labels, dtype=np.float64)
+        n = X.shape[0]
+        w = np.zeros(self.NF)
+        hist = []
+        for _ in range(iters):
+            s = X @ w
+            q = 0.5 * (1.0 + np.tanh(0.5 * s))                       # sigmoid
+            hist.append(float(np.mean(np.logaddexp(0.0, s) - y * s)))
+            g = X.T @ (y - q) - ridge * n * w
+            H = (X * (q * (1.0 - q))[:, None]).T @ X + ridge * n * np.eye(self.NF)
+            step = np.linalg.solve(H, g)
+            w = w + step
+            if np.max(np.abs(step)) < 1e-10:
+                break
+        self.w = w
+        return hist
+
+    def __call__(self, z, p):
+        return self._features(z, p) @ self.w
+
+    def state_dict(self):
+        return {'w': self.w.tolist(), 'mean': self.mean.tolist(), 'std': self.std.tolist()}

@@ -6,9 +6,11 @@
 Data: pirate_amd.scorer_data (the 1024-channel rank-10 chain, 16 beams per chunk, simulated
 pulses in half of the beams, noise + RFI everywhere). Training and evaluation use disjoint
 pulse banks and noise streams. Prints progress lines and one JSON summary: training loss per
-epoch, completeness of the scorer and of the peak finder at false-alarm rates 1e-2, 1e-3 and
-1e-4 on evaluation negatives with RFI and, separately, on pure-noise evaluation data (every
-beam without RFI), overall and per SNR bin, the data-generation rate, and the
+epoch, completeness of the scorer, of the scorer with its head (pirate_amd.ScoreHead: the
+network's logit and the peak finder's statistic combined by a logistic regression fitted on
+the training tiles) and of the peak finder at false-alarm rates 1e-2, 1e-3 and 1e-4 on
+evaluation negatives with RFI and, separately, on pure-noise evaluation data (every beam
+without RFI), overall and per SNR bin, the data-generation rate, and the
 scorer kernels' throughput measured with HIP events (forward: tiles/s and bf16 MFMA TFLOP/s at
 2 * 16 * 256 * (64 + 16 + 16) * 9 flop per tile; training step: tiles/s).
 """
@@ -66,6 +68,8 @@ def main(argv=None):
     ap.add_argument('--eval-bank', type=int, default=128)
     ap.add_argument('--eval-chunks', type=int, default=400)
     ap.add_argument('--seed', type=int, default=5)
+    ap.add_argument('--head-clean-chunks', type=int, default=100,
+                    help='chunks of pure-noise tiles (no RFI) added to the head fit')
     ap.add_argument('--nproc', type=int, default=None, help='worker processes for the pulse bank (1: none)')
     ap.add_argument('--out', default=None, help='save the trained parameters (.npz)')
     ap.add_argument('--json', default=None, help='also write the summary here')
@@ -97,24 +101,41 @@ def main(argv=None):
     t_train = time.perf_counter() - t0
     rates = kernel_rates(scr, train['tiles'][:65536].contiguous(), train['labels'][:65536].contiguous())
     log(f'kernels: {rates}')
+    # the head over (network logit, peak-finder statistic), fitted on the training tiles
+    # (on the training tiles plus pure-noise tiles of the training pulse bank, so that it sees
+    # negatives both with and without RFI)
+    zs, ps, ys = [sd.score(scr, train['tiles'])], [train['pf']], [train['labels'].cpu().numpy()]
+    if a.head_clean_chunks > 0:
+        clean = sd.build_dataset(sd.TileStream(bank, nbeams=16, seed=a.seed + 3, neg_per_beam=8, rfi=False),
+                                 a.head_clean_chunks)
+        zs.append(sd.score(scr, clean['tiles']))
+        ps.append(clean['pf'])
+        ys.append(clean['labels'].cpu().numpy())
+        del clean
+    head = pa.ScoreHead()
+    head_loss = head.fit(np.concatenate(zs), np.concatenate(ps), np.concatenate(ys))
+    log(f'head: log-loss {head_loss[0]:.4f} -> {head_loss[-1]:.4f}, weights {np.round(head.w, 3).tolist()}')
     del train
 
     def evaluate(ev, what):
         s = sd.score(scr, ev['tiles'])
         y = ev['labels'].cpu().numpy() > 0.5
+        sh = head(s, ev['pf'])
         res = {'pulses': int(y.sum()), 'negatives': int((~y).sum()), 'far': {}}
         for far in (1e-2, 1e-3, 1e-4):
             c_nn, thr_nn = sd.completeness_at_far(s[y], s[~y], far)
+            c_hd, _ = sd.completeness_at_far(sh[y], sh[~y], far)
             c_pf, thr_pf = sd.completeness_at_far(ev['pf'][y], ev['pf'][~y], far)
             bins = {}
             for lo, hi in ((5, 10), (10, 15), (15, 20), (20, 30)):
                 m = (ev['snr'][y] >= lo) & (ev['snr'][y] < hi)
                 bins[f'snr_{lo}_{hi}'] = {'scorer': float(np.mean(s[y][m] > thr_nn)),
                                           'peak_finder': float(np.mean(ev['pf'][y][m] > thr_pf)), 'n': int(m.sum())}
-            res['far'][f'{far:g}'] = {'scorer': c_nn, 'peak_finder': c_pf, 'scorer_threshold_logit': thr_nn,
-                                      'peak_finder_threshold_snr': thr_pf, 'by_snr': bins}
-            log(f'{what}: FAR {far:g}: completeness scorer {c_nn:.3f}, peak finder {c_pf:.3f} '
-                f'(peak-finder threshold SNR {thr_pf:.1f})')
+            res['far'][f'{far:g}'] = {'scorer': c_nn, 'scorer_with_head': c_hd, 'peak_finder': c_pf,
+                                      'scorer_threshold_logit': thr_nn, 'peak_finder_threshold_snr': thr_pf,
+                                      'by_snr': bins}
+            log(f'{what}: FAR {far:g}: completeness scorer {c_nn:.3f}, scorer + head {c_hd:.3f}, '
+                f'peak finder {c_pf:.3f} (peak-finder threshold SNR {thr_pf:.1f})')
         return res
 
     ev = sd.build_dataset(sd.TileStream(ebank, nbeams=16, seed=a.seed + 1, neg_per_beam=128), a.eval_chunks, log=log)
@@ -127,9 +148,10 @@ def main(argv=None):
     summary = {'train_tiles': n, 'epochs': a.epochs, 'batch': a.batch, 'lr': a.lr, 'loss_per_epoch': hist,
                'bank': len(bank), 'eval_bank': len(ebank), 'bank_s': t_bank, 'data_s': t_data,
                'data_tiles_per_s': n / t_data, 'train_s': t_train, 'kernels': rates,
+               'head': head.state_dict(), 'head_log_loss': head_loss,
                'eval_noise_rfi': res_rfi, 'eval_noise_only': res_clean}
     if a.out:
-        np.savez(a.out, params=scr.params.cpu().numpy())
+        np.savez(a.out, params=scr.params.cpu().numpy(), head_w=head.w, head_mean=head.mean, head_std=head.std)
     print(json.dumps(summary), flush=True)
     if a.json:
         with open(a.json, 'w') as f:

@@ -9,8 +9,9 @@ streams.
 
 The peak finder's statistic for a tile is the maximum of its out_max over the tile's region
 (bare-kernel weights: a boxcar SNR). At a false-alarm rate of 1e-3 on the evaluation
-negatives, the trained scorer's completeness must be at least the peak finder's, and the
-training loss must fall. This is a reduced run (about 77k training tiles, 3 epochs); the full
+negatives, the trained scorer's completeness, alone and with its head (pirate_amd.ScoreHead,
+fitted on the training tiles), must be at least the peak finder's, and the training loss
+must fall. This is a reduced run (about 77k training tiles, 3 epochs); the full
 10^6-tile run is python -m pirate_amd.train_scorer."""
 
 import sys
@@ -47,16 +48,22 @@ def test_trained_scorer_completeness_vs_peak_finder():
     scr = pa.LearnedScorer(seed=5)
     hist = sd.train(scr, train, epochs=3, batch=512, lr=2e-3, seed=5, log=log)
     assert hist[-1] < 0.8 * hist[0], hist
+    head = pa.ScoreHead()
+    hl = head.fit(sd.score(scr, train['tiles']), train['pf'], train['labels'].cpu().numpy())
+    assert hl[-1] < hl[0]
     del train
     log('evaluation tiles')
     ev = sd.build_dataset(sd.TileStream(bank_eval, nbeams=16, seed=6, neg_per_beam=128), 120, log=log)
     s = sd.score(scr, ev['tiles'])
     y = ev['labels'].cpu().numpy() > 0.5
     assert np.all(np.isfinite(s))
+    sh = head(s, ev['pf'])
     for far in (1e-2, 1e-3):
         c_nn, thr_nn = sd.completeness_at_far(s[y], s[~y], far)
+        c_hd, _ = sd.completeness_at_far(sh[y], sh[~y], far)
         c_pf, thr_pf = sd.completeness_at_far(ev['pf'][y], ev['pf'][~y], far)
         log(f'FAR {far:g} over {int((~y).sum())} negatives: completeness scorer {c_nn:.3f} (threshold {thr_nn:.2f}), '
-            f'peak finder {c_pf:.3f} (threshold {thr_pf:.2f} SNR) over {int(y.sum())} pulses')
+            f'scorer + head {c_hd:.3f}, peak finder {c_pf:.3f} (threshold {thr_pf:.2f} SNR) over {int(y.sum())} pulses')
         if far == 1e-3:
             assert c_nn >= c_pf
+            assert c_hd >= c_pf
