@@ -232,31 +232,50 @@ def gather_tiles(source, origins):
     return rows[o[:, 0:1, None], r, t].to(torch.bfloat16)
 
 
-def build_dataset(stream, nchunks, log=None):
-    """Runs nchunks chunks of a TileStream; returns a dict of device tiles (n, 64, 256) bf16,
-    labels (n,) float32 and numpy arrays pf (n,), snr, dm, width_ms, tau600_ms (NaN for
-    negatives)."""
+def _tiles_per_chunk(stream):
+    """Upper bound of the tiles one chunk of a TileStream yields (pulse tiles + negatives)."""
+    return stream.npulse_beams * (stream.T // TILE_T) + (stream.B - stream.npulse_beams) * stream.neg_per_beam
+
+
+def build_dataset(stream, nchunks, log=None, extra=()):
+    """Runs nchunks chunks of a TileStream (then nchunks' chunks of each (stream', nchunks') in
+    extra); returns a dict of device tiles (n, 64, 256) bf16, labels (n,) float32 and numpy
+    arrays pf (n,), snr, dm, width_ms, tau600_ms (NaN for negatives). The tiles go straight
+    into one preallocated tensor (no concatenation, whose copy would double the peak memory of
+    a dataset of 10^6 tiles or more)."""
     import torch
-    tiles, labels, pf, meta = [], [], [], []
-    for i in range(nchunks):
-        r = stream.step()
-        for kind in ('pos', 'neg'):
-            o = r[kind]
-            if len(o) == 0:
-                continue
-            tiles.append(gather_tiles(r['source'], o))
-            labels.append(np.full(len(o), 1.0 if kind == 'pos' else 0.0, dtype=np.float32))
-            pf.append(r['pf_' + kind])
-            if kind == 'pos':
-                meta.append(np.stack([r['snr'], r['dm'], r['width_ms'], r['tau600_ms']], axis=1))
-            else:
-                meta.append(np.full((len(o), 4), np.nan))
-        if log and (i + 1) % max(1, nchunks // 10) == 0:
-            log(f'dataset: {i + 1}/{nchunks} chunks, {sum(len(x) for x in labels)} tiles')
+    jobs = [(stream, nchunks)] + list(extra)
+    cap = sum(n * _tiles_per_chunk(s) for s, n in jobs)
+    tiles = torch.empty((cap, TILE_R, TILE_T), dtype=torch.bfloat16, device='cuda')
+    k = 0
+    labels, pf, meta = [], [], []
+    ntot = sum(n for _, n in jobs)
+    done = 0
+    for st, n in jobs:
+        for i in range(n):
+            r = st.step()
+            for kind in ('pos', 'neg'):
+                o = r[kind]
+                if len(o) == 0:
+                    continue
+                if k + len(o) > cap:
+                    raise RuntimeError('build_dataset: more tiles than the per-chunk bound')
+                tiles[k:k + len(o)] = gather_tiles(r['source'], o)
+                k += len(o)
+                labels.append(np.full(len(o), 1.0 if kind == 'pos' else 0.0, dtype=np.float32))
+                pf.append(r['pf_' + kind])
+                if kind == 'pos':
+                    meta.append(np.stack([r['snr'], r['dm'], r['width_ms'], r['tau600_ms']], axis=1))
+                else:
+                    meta.append(np.full((len(o), 4), np.nan))
+            done += 1
+            if log and done % max(1, ntot // 10) == 0:
+                log(f'dataset: {done}/{ntot} chunks, {k} tiles')
     torch.cuda.synchronize()
     meta = np.concatenate(meta) if meta else np.zeros((0, 4))
-    return dict(tiles=torch.cat(tiles), labels=torch.from_numpy(np.concatenate(labels)).cuda(),
-                pf=np.concatenate(pf), snr=meta[:, 0], dm=meta[:, 1], width_ms=meta[:, 2], tau600_ms=meta[:, 3])
+    return dict(tiles=tiles[:k], labels=torch.from_numpy(np.concatenate(labels) if labels else np.zeros(0, np.float32)).cuda(),
+                pf=np.concatenate(pf) if pf else np.zeros(0), snr=meta[:, 0], dm=meta[:, 1], width_ms=meta[:, 2],
+                tau600_ms=meta[:, 3])
 
 
 def train(scorer, data, epochs=1, batch=512, lr=2e-3, seed=0, log=None):

@@ -1,10 +1,11 @@
 """Trains the learned peak scorer on the GPU and compares it with the plain peak finder.
 
-    python -m pirate_amd.train_scorer [--train-tiles 1000000] [--epochs 2] [--eval-chunks 400]
+    python -m pirate_amd.train_scorer [--train-tiles 4000000] [--epochs 6] [--clean-frac 0.5] [--eval-chunks 400]
                                       [--out scorer_params.npz] [--json summary.json]
 
 Data: pirate_amd.scorer_data (the 1024-channel rank-10 chain, 16 beams per chunk, simulated
-pulses in half of the beams, noise + RFI everywhere). Training and evaluation use disjoint
+pulses in half of the beams, noise + RFI everywhere in a fraction 1 - clean_frac of the
+training chunks, pure noise in the rest). Training and evaluation use disjoint
 pulse banks and noise streams. Prints progress lines and one JSON summary: training loss per
 epoch, completeness of the scorer, of the scorer with its head (pirate_amd.ScoreHead: the
 network's logit and the peak finder's statistic combined by a logistic regression fitted on
@@ -60,14 +61,16 @@ def kernel_rates(scorer, tiles, labels, reps=5):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
-    ap.add_argument('--train-tiles', type=int, default=1000000)
-    ap.add_argument('--epochs', type=int, default=2)
+    ap.add_argument('--train-tiles', type=int, default=4000000)
+    ap.add_argument('--epochs', type=int, default=6)
     ap.add_argument('--batch', type=int, default=512)
     ap.add_argument('--lr', type=float, default=2e-3)
     ap.add_argument('--bank', type=int, default=512, help='simulated pulses in the training bank')
     ap.add_argument('--eval-bank', type=int, default=128)
     ap.add_argument('--eval-chunks', type=int, default=400)
     ap.add_argument('--seed', type=int, default=5)
+    ap.add_argument('--clean-frac', type=float, default=0.5,
+                    help='fraction of the training chunks drawn without RFI (pure noise + pulses)')
     ap.add_argument('--head-clean-chunks', type=int, default=100,
                     help='chunks of pure-noise tiles (no RFI) added to the head fit')
     ap.add_argument('--nproc', type=int, default=None, help='worker processes for the pulse bank (1: none)')
@@ -88,7 +91,10 @@ def main(argv=None):
     per_chunk = stream.npulse_beams * (stream.T // sd.TILE_T) + (stream.B - stream.npulse_beams) * 8
     nchunks = -(-a.train_tiles // per_chunk) + 1
     t0 = time.perf_counter()
-    train = sd.build_dataset(stream, nchunks, log=log)
+    nclean = int(round(a.clean_frac * nchunks))
+    extra = [(sd.TileStream(bank, nbeams=16, seed=a.seed + 4, neg_per_beam=8, rfi=False), nclean)] if nclean else []
+    train = sd.build_dataset(stream, nchunks - nclean, log=log, extra=extra)
+    del extra
     t_data = time.perf_counter() - t0
     n = train['tiles'].shape[0]
     log(f'{n} training tiles from {nchunks} chunks in {t_data:.1f} s')
@@ -145,7 +151,7 @@ def main(argv=None):
                           a.eval_chunks, log=log)
     res_clean = evaluate(ev, 'noise only')
     del ev
-    summary = {'train_tiles': n, 'epochs': a.epochs, 'batch': a.batch, 'lr': a.lr, 'loss_per_epoch': hist,
+    summary = {'train_tiles': n, 'clean_frac': a.clean_frac, 'epochs': a.epochs, 'batch': a.batch, 'lr': a.lr, 'loss_per_epoch': hist,
                'bank': len(bank), 'eval_bank': len(ebank), 'bank_s': t_bank, 'data_s': t_data,
                'data_tiles_per_s': n / t_data, 'train_s': t_train, 'kernels': rates,
                'head': head.state_dict(), 'head_log_loss': head_loss,

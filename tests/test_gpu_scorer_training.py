@@ -12,7 +12,7 @@ The peak finder's statistic for a tile is the maximum of its out_max over the ti
 negatives, the trained scorer's completeness, alone and with its head (pirate_amd.ScoreHead,
 fitted on the training tiles), must be at least the peak finder's, and the training loss
 must fall. This is a reduced run (about 77k training tiles, 3 epochs); the full
-10^6-tile run is python -m pirate_amd.train_scorer."""
+4 x 10^6-tile run is python -m pirate_amd.train_scorer."""
 
 import sys
 
