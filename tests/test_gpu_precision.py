@@ -59,7 +59,7 @@ def test_bfloat16_rms_vs_fp64_oracle():
 
 
 @pytest.mark.parametrize('dtype', ['float16', 'bfloat16'])
-@pytest.mark.parametrize('name', ['toy', 'zones4', 'subbands', 'cfg1'])
+@pytest.mark.parametrize('name', ['toy', 'zones4', 'subbands', 'cfg1', 'pri5'])
 def test_16bit_storage_bit_exact_vs_rounding_oracle(name, dtype):
     """With 16-bit storage the GPU rounds to the storage type only where it stores (gridded
     rows, downsampled rows, lag ring, subband rows, out_max) and computes in float32 in
