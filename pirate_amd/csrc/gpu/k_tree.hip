@@ -312,9 +312,22 @@ k_stage2(Stage2Args a)
                     }
                 }
             } else {
-                for (int fc = wid; fc < TD::D; fc += kTreeThreads / 64) {
-                    const int4 d = desc[fc];
-                    tile[fc * kTT + lane] = ld_st(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0)));
+                // each wave's rows: all loads issued, then converted and written to the tile
+                constexpr int NWV = kTreeThreads / 64, RPW = (TD::D + NWV - 1) / NWV;
+                raw_st<Tst> vr[RPW];
+#pragma unroll
+                for (int k = 0; k < RPW; k++) {
+                    const int fc = wid + k * NWV;
+                    if (fc < TD::D) {
+                        const int4 d = desc[fc];
+                        vr[k] = ld_raw(ring + (d.x + tl - ((tl >= d.y) ? d.z : 0)));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < RPW; k++) {
+                    const int fc = wid + k * NWV;
+                    if (fc < TD::D)
+                        tile[fc * kTT + lane] = from_raw<Tst>(vr[k]);
                 }
             }
             __syncthreads();
