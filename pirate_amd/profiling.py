@@ -28,8 +28,13 @@ def _rows(path):
         return list(csv.DictReader(f))
 
 
+# Optional substring that a library dispatch's kernel name must also contain (e.g. '__half'
+# to keep the float16 chain of a run that holds float32 and float16 chains), set by --only.
+_ONLY = ''
+
+
 def _is_lib(name):
-    return LIB_PREFIX in name
+    return LIB_PREFIX in name and _ONLY in name
 
 
 def _short(name):
@@ -128,7 +133,12 @@ def main(argv=None):
     ap.add_argument('--command', action='append', default=[], help='command lines to record')
     ap.add_argument('--md', required=True)
     ap.add_argument('--json', required=True)
+    ap.add_argument('--only', default='', help='keep the library dispatches whose kernel name contains this '
+                                                '(e.g. __half: the float16 chain of a run with both storage types)')
+    ap.add_argument('--alg-key', default=None, help='JSON path of the per-kernel table in --alg-json (e.g. fp16)')
     a = ap.parse_args(argv)
+    global _ONLY
+    _ONLY = a.only
 
     tp = [int(x) for x in a.tree_primary.split(',')] if a.tree_primary else None
     names = chain_names(a.npri, a.ntrees, tp)
@@ -139,7 +149,8 @@ def main(argv=None):
             for line in f:
                 line = line.strip()
                 if line.startswith('{'):
-                    cfg = json.loads(line)['config']
+                    rec = json.loads(line)
+                    cfg = rec[a.alg_key] if a.alg_key else rec['config']
                     pk = cfg.get('per_kernel_serialized', cfg.get('per_kernel', {}))
                     alg = {k: v['alg_bytes'] for k, v in pk.items()}
     fetch = write = None
