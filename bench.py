@@ -317,7 +317,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f'bench.py: this build measures one GPU only (--gpus {args.gpus} requested). The chain shards '
+        print(f'bench.py measures one GPU only (--gpus {args.gpus} requested). The chain shards '
               'by beam with no communication; run one process per GPU.', file=sys.stderr)
         sys.exit(2)
 

@@ -256,7 +256,7 @@ int pdd_dedisperser_get_tuning(const pdd_dedisperser *dd, const char *name, int6
  * (the caller fills chunk c+1 while chunk c runs). Per chunk, every member's chain runs on a
  * branch stream of its own, forked from and joined into one launch stream, so that the
  * members' kernels run at once. use_graph must be 0: HIP graph replay of the chunk is refused
- * in this build (its capture path ended in a segmentation fault on MI355X, cause not yet
+ * in this library (its capture path ended in a segmentation fault on MI355X, cause not yet
  * identified); pdd_group_create and pdd_group_set_use_graph fail for use_graph != 0. */
 typedef struct pdd_group pdd_group;
 typedef struct {

@@ -254,10 +254,10 @@ void GpuDedisperser::validate_gpu_limits()
     PDD_CHECK(nt_in % 4 == 0, "GpuDedisperser: nt_in must be a multiple of 4");
     for (long p = 0; p < npri; p++) {
         long Tp = nt_in >> p;
-        PDD_CHECK(Tp % 32 == 0, "GpuDedisperser: this build needs nt_in/2^p to be a multiple of 32 for every primary tree p (nt_in="
+        PDD_CHECK(Tp % 32 == 0, "GpuDedisperser: pirate_amd needs nt_in/2^p to be a multiple of 32 for every primary tree p (nt_in="
                   << nt_in << ", p=" << p << ")");
         PDD_CHECK(plan->stage1_dd_rank[p] <= kGpuMaxDD && plan->stage1_dd_rank[p] >= 1,
-                  "GpuDedisperser: stage-1 dd_rank " << plan->stage1_dd_rank[p] << " not supported by this build (max " << kGpuMaxDD << ")");
+                  "GpuDedisperser: stage-1 dd_rank " << plan->stage1_dd_rank[p] << " not supported by pirate_amd (max " << kGpuMaxDD << ")");
         if (p > 0) {
             long s = plan->stage1_dd_rank[1] + 1;
             PDD_CHECK(pow2(s - 1) <= Tp, "GpuDedisperser: lagged-downsampling lag exceeds the chunk at p=" << p);
@@ -267,7 +267,7 @@ void GpuDedisperser::validate_gpu_limits()
     for (long it = 0; it < ntrees; it++) {
         const DedispersionTree &t = plan->trees[it];
         const TreeTables &T = tables[it];
-        PDD_CHECK(t.dd_rank <= kGpuMaxDD, "GpuDedisperser: stage-2 dd_rank " << t.dd_rank << " not supported by this build (max " << kGpuMaxDD << ")");
+        PDD_CHECK(t.dd_rank <= kGpuMaxDD, "GpuDedisperser: stage-2 dd_rank " << t.dd_rank << " not supported by pirate_amd (max " << kGpuMaxDD << ")");
         if (!T.fast && t.dd_rank <= kGpuMaxDDGeneral)
             PDD_CHECK(stage2_lds_bytes(t.dd_rank, T.stag_hist_nelts) <= kLdsBytesPerCU,
                       "GpuDedisperser: tree " << it << " needs too much LDS (subband history)");
@@ -1202,7 +1202,7 @@ void GpuDedisperser::output_ipc_handle(void *handle, size_t *nbytes)
 {
     PDD_CHECK(is_allocated, "output_ipc_handle(): allocate() has not been called");
     PDD_CHECK(out_region.nbytes < (size_t(1) << 31),
-              "output_ipc_handle(): the output ring is " << out_region.nbytes << " bytes; this build exports at most 2 GiB");
+              "output_ipc_handle(): the output ring is " << out_region.nbytes << " bytes; pirate_amd exports at most 2 GiB");
     static_assert(sizeof(hipIpcMemHandle_t) == 64, "IPC handle size");
     HIP_CHECK(hipSetDevice(params.device));
     hipIpcMemHandle_t h;

@@ -21,7 +21,7 @@ void launch_advance_counter(long *counter, hipStream_t st);   // k_group.hip
 // cause has not been identified. The chunk's launches are issued one by one.
 static void refuse_graph()
 {
-    throw pdd_error("GpuDedisperserGroup: HIP graph replay is disabled in this build (a segmentation fault in "
+    throw pdd_error("GpuDedisperserGroup: HIP graph replay is disabled in pirate_amd (a segmentation fault in "
                     "its capture path is not yet explained); use use_graph = false");
 }
 

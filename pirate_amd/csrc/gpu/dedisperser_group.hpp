@@ -9,7 +9,7 @@
 // is issued on a branch stream of its own, forked from and joined into one launch stream, so
 // the members' kernels share the device at once. The kernels can read the chunk index from a
 // device counter that the chunk's last node advances (for replay of a captured HIP graph, one
-// per (input slot, output slot, weight slot) phase); graph replay is refused in this build
+// per (input slot, output slot, weight slot) phase); graph replay is refused
 // (see dedisperser_group.cpp), and the launches are issued one by one.
 //
 // Per-chunk protocol (one producer, one consumer, chunks in order):

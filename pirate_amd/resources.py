@@ -5,7 +5,7 @@ DedispersionKernel.cu:515-516, CoalescedDdKernel2.cu:131-140).
 
 Counts are per launch of one beam batch (one chunk of beams_per_batch beams). Two models:
 
-* algorithmic ("alg"): the compulsory HBM reads and writes of each kernel of this build's
+* algorithmic ("alg"): the compulsory HBM reads and writes of each kernel of this library's
   chain -- the fused gridding + lagged downsampling reads the input chunk and writes the
   gridded rows and every downsampled level; stage 1 reads its input rows and writes the lag
   ring; stage 2 reads the lag ring and writes the peak-finder rows; the peak finder reads

@@ -15,7 +15,7 @@ primary tree (a second one would need a stage-1 kernel of rank 0, which
 GpuDedispersionKernel rejects: src_lib/DedispersionKernel.cu:496).
 
 Every draw comes from a numpy Generator, so a seed names a config. gpu_supported() reports
-whether this build's GPU chain accepts a config, and the reason if not.
+whether pirate_amd's GPU chain accepts a config, and the reason if not.
 """
 
 import math
@@ -169,7 +169,7 @@ def make_random_config(rng, max_toplevel_rank=10, max_early_triggers=5, dtypes=(
 
 
 def gpu_supported(cfg_dict):
-    """(True, '') if this build's GPU chain accepts the config, else (False, reason). Builds
+    """(True, '') if pirate_amd's GPU chain accepts the config, else (False, reason). Builds
     the plan and the dedisperser object (no GPU memory is touched before allocate())."""
     import pirate_amd as pa
     try:

@@ -229,7 +229,7 @@ def test_steady_state_it0_closed_form():
 def test_reference_configs_accepted_and_limits_reported_cleanly():
     # Every config of the reference's configs/dedispersion/ (ranks up to 16, subband
     # searches, early triggers) builds a GPU dedisperser (no device memory is touched before
-    # allocate()); a shape beyond this build's kernels (peak-finder time downsampling 64)
+    # allocate()); a shape beyond pirate_amd's kernels (peak-finder time downsampling 64)
     # fails at creation with a clear message, not at launch time.
     for name, d in golden()['configs'].items():
         pa.GpuDedisperser(pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d)))

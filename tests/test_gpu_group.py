@@ -1,6 +1,6 @@
 """Heterogeneous beam groups (GpuDedisperserGroup): every member's chain of a chunk runs on a
 branch stream of its own, forked from and joined into one launch stream. (HIP graph replay of
-the chunk is refused in this build; its test case is skipped with the reason.)
+the chunk is refused in pirate_amd; its test case is skipped with the reason.)
 
 Checks, bit for bit:
   * a small group of three shapes (ranks 8/9/10, 2/3/4 primary trees, float32 and float16
@@ -59,7 +59,7 @@ def host_outputs(out, b):
             [a[b:b + 1].cpu().numpy().view(np.uint32) for a in out.out_argmax])
 
 
-GRAPH_SKIP = pytest.mark.skip(reason='HIP graph replay of the group chain is refused in this build: its first run '
+GRAPH_SKIP = pytest.mark.skip(reason='HIP graph replay of the group chain is refused in pirate_amd: its first run '
                                      'ended in a segmentation fault inside release_input_and_launch whose cause is '
                                      'not identified')
 

@@ -1,6 +1,6 @@
 """The random config generator (the reference's DedispersionConfig::make_random()) on the
 CPU: every draw is a valid config and plan, its subband-count restriction agrees with the
-plan's per-tree subband counts, and this build's GPU chain accepts every draw (the
+plan's per-tree subband counts, and pirate_amd's GPU chain accepts every draw (the
 dedisperser object is built without touching GPU memory)."""
 
 import numpy as np
