@@ -9,7 +9,7 @@ Per-level scaling: every tree pass scales by 1/sqrt(2), so every level's values 
 variance (white-noise input) and float16 stays far inside its range; a power-of-two scale
 cannot change a floating-point type's relative rounding, which alone sets the error below.
 
-Bound: the north star's 1% of SNR (relative to max(|SNR|, 1)) for the maximum, 2e-3 / 1e-2
+Bound: 1% of SNR (relative to max(|SNR|, 1)) for the maximum, 2e-3 / 1e-2
 for the rms. float16 meets it (measured max ~3e-3). bfloat16 has an 8-bit significand: each
 of the ~4 storage roundings on the path is up to 2^-9 relative, and the measured maximum is
 ~2e-2, so its maximum-error case is marked xfail (its rms meets 1e-2). The GPU's argmax
