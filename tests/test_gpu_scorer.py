@@ -180,13 +180,14 @@ def test_tree_rows_of_a_dedisperser():
     np.testing.assert_array_equal(got, want)
 
 
-def test_scorer_in_chain():
+@pytest.mark.parametrize('dtype', ['float32', 'float16'])
+def test_scorer_in_chain(dtype):
     """GpuDedisperser.set_scorer: the chain scores every chunk's tiles of tree 0 right after its
     stage 2; the logits equal LearnedScorer.forward on the same tiles of the tree rows."""
     import torch
     import pirate_amd as pa
     from gpu_cases import CASES
-    d = dict(CASES['cfg1'], beams_per_gpu=4, beams_per_batch=2)
+    d = dict(CASES['cfg1'], beams_per_gpu=4, beams_per_batch=2, dtype=dtype)
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
     dd = pa.GpuDedisperser(plan)
     dd.allocate()
@@ -203,7 +204,8 @@ def test_scorer_in_chain():
             assert tuple(out.out_score.shape) == (2, 16, 8)
             got = out.out_score.cpu().numpy().reshape(-1)
             src = dd.tree_rows(0)
-            src.struct.base = src.struct.base + out.ibeam * src.struct.beam_stride * 4
+            esize = 4 if dtype == 'float32' else 2
+            src.struct.base = src.struct.base + out.ibeam * src.struct.beam_stride * esize
             want = scr.forward(src.with_origins(grid)).cpu().numpy()
         np.testing.assert_array_equal(got, want)
         assert np.all(np.isfinite(got))
