@@ -17,7 +17,8 @@ all 96 beams, each group's chain on a branch stream of its own.
 slots (HBM) before the timed region. Per-kernel times come from HIP events around each kernel
 in a separate pass with every kernel run alone on one stream (launch shape fork=2), and the
 roofline object reports the dominant kernel's algorithmic bytes / average duration against
-MI355X's 8 TB/s HBM peak; its "traffic" (HBM bytes per launch) comes from the rocprofv3
+MI355X's 8 TB/s HBM peak, and beside it against a plain 1 GiB device copy timed in the same
+run (device_copy_GBps); its "traffic" (HBM bytes per launch) comes from the rocprofv3
 FETCH_SIZE / WRITE_SIZE summary committed in profiles/ for this workload. The CPU baseline runs
 the oracle (a single-threaded C++ restatement of the reference's CPU dedisperser) on bounded
 samples of the same shapes: one beam per thread on all host cores, one core, and the 1-beam
@@ -168,6 +169,25 @@ def measured_traffic(kernel):
         return int(rec['hbm_read_bytes'] + rec['hbm_write_bytes'])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def device_copy_gbps(torch, nbytes=2**30, reps=10):
+    """HBM bytes (read + write) per second of a plain nbytes device-to-device copy (torch's copy
+    kernel, on torch's current stream, timed with events on that stream): the practical ceiling
+    that an HBM-bound kernel is compared with beside the published peak."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 # ---------------------------------------------------------------------------------------------
@@ -348,6 +368,8 @@ def main():
     for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]['avg_ms']):
         log(f'kernel {k:12s} {v["avg_ms"]:8.3f} ms  {v["GBps"]:8.1f} GB/s (alg bytes {v["alg_bytes"] / 1e6:.1f} MB)')
 
+    copy_gbps = device_copy_gbps(torch)
+    log(f'plain 1 GiB device copy: {copy_gbps:.1f} GB/s (read + write)')
     chunk_s = T * cfg.time_sample_ms * 1e-3
     value = B * chunk_s / dt
     chain_gbps = chain_alg_bytes(plan) / dt / 1e9
@@ -378,7 +400,9 @@ def main():
                    'real_time_beams_capacity_capped': round(min(value, HBM_GIB // gib_per_beam), 1)},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': per_kernel[dom]['GBps'], 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': round(per_kernel[dom]['GBps'] / HBM_PEAK_GBPS, 4),
-                     'traffic': measured_traffic(dom)},
+                     'traffic': measured_traffic(dom),
+                     'device_copy_GBps': round(copy_gbps, 1),
+                     'frac_of_device_copy': round(per_kernel[dom]['GBps'] / copy_gbps, 4)},
     }
     if delivery:
         result['config']['with_input_delivery'] = dict(
