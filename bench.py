@@ -137,27 +137,27 @@ def cpu_timing(cfg_dict, nthreads, nchunks):
 
 
 def cpu_baseline():
-    from gpu_cases import CASES, cfg2_with_beams
+    from gpu_cases import CASES, chord16k_with_beams
     model, ncpu = host_cpu()
     nthr = max(1, min(ncpu, 16))
-    d = cfg2_with_beams(1)
+    d = chord16k_with_beams(1)
     chunk_s = d['time_samples_per_chunk'] * d['time_sample_ms'] * 1e-3
     t1, _ = cpu_timing(d, 1, 12)
     log(f'cpu baseline: chord16k shapes, 1 thread: {t1:.3f} s per beam-chunk')
     tn, wall = cpu_timing(d, nthr, 4)
     log(f'cpu baseline: chord16k shapes, {nthr} threads: {tn:.3f} s per beam-chunk per thread ({wall:.1f} s wall)')
-    d1 = dict(CASES['cfg1'])
+    d1 = dict(CASES['r10_1024ch'])
     c1, _ = cpu_timing(d1, 1, 40)
     chunk1 = d1['time_samples_per_chunk'] * d1['time_sample_ms'] * 1e-3
-    log(f'cpu baseline: cfg1 (1024 ch, rank 10, 1 tree), 1 thread: {c1 * 1e3:.1f} ms per beam-chunk')
+    log(f'cpu baseline: 1024 ch, rank 10, 1 tree, 1 thread: {c1 * 1e3:.1f} ms per beam-chunk')
     return dict(value=round(nthr * chunk_s / tn, 4), unit='beams', cores=nthr, kind='port',
                 sample=f'chord16k shapes (16384 ch, rank 14, 4 trees, T=2048, fp32): {nthr} threads x 1 beam x 4 chunks '
                        f'(after a warm-up chunk), {tn:.3f} s per beam-chunk per thread, on {nthr} of {ncpu} usable '
                        f'cores of "{model}"',
                 single_core=dict(value=round(chunk_s / t1, 4), cores=1, s_per_beam_chunk=round(t1, 4),
                                  sample='1 beam x 12 chunks of the chord16k shapes'),
-                cfg1=dict(value=round(chunk1 / c1, 2), cores=1, ms_per_beam_chunk=round(c1 * 1e3, 3),
-                          sample='1 beam, 1024 ch 400-800 MHz, rank 10, 1 tree, T=2048, fp32, 40 chunks'),
+                r10_1024ch=dict(value=round(chunk1 / c1, 2), cores=1, ms_per_beam_chunk=round(c1 * 1e3, 3),
+                                sample='1 beam, 1024 ch 400-800 MHz, rank 10, 1 tree, T=2048, fp32, 40 chunks'),
                 host_cpu=model, host_cpus_usable=ncpu)
 
 
@@ -258,9 +258,9 @@ def measure(pa, torch, cfg_dict, steps, warmup, timing_steps):
 
 def measure_batched(pa, torch, steps, warmup, timing_steps):
     """The 96-beam heterogeneous group, every group's chain on its own branch stream, resident inputs."""
-    from gpu_cases import cfg4_groups
+    from gpu_cases import batched_groups
     from pirate_amd.resources import chain_alg_bytes
-    cfgs = cfg4_groups(32)
+    cfgs = batched_groups(32)
     plans = [pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d)) for d in cfgs]
     g = pa.GpuDedisperserGroup(plans)
     g.allocate()
@@ -343,7 +343,7 @@ def main():
 
     import torch
     import pirate_amd as pa
-    from gpu_cases import CFG2
+    from gpu_cases import CHORD16K
     from pirate_amd.resources import chain_alg_bytes
 
     if args.calib_copy:
@@ -354,7 +354,7 @@ def main():
         del src, dst
         log('calibration copy: 1 GiB read + 1 GiB write')
 
-    plan, dt, per_kernel, gib, dd = measure(pa, torch, CFG2, args.steps, args.warmup, args.timing_steps)
+    plan, dt, per_kernel, gib, dd = measure(pa, torch, CHORD16K, args.steps, args.warmup, args.timing_steps)
     delivery = None
     if not args.no_input_delivery:
         # the reference's time() loop: int4 upload + dequantization inside every step
@@ -429,7 +429,7 @@ def main():
         log(f'fp16 batched: {v16:.1f} real-time beams ({v16 / value:.2f}x fp32 plain), chain {alg_gbps:.0f} GB/s alg')
     if not args.no_fp16:
         # chord16k shapes with float16 storage (float32 arithmetic)
-        p16, dt16, pk16, gib16, dd16 = measure(pa, torch, dict(CFG2, dtype='float16'), max(args.steps // 2, 1),
+        p16, dt16, pk16, gib16, dd16 = measure(pa, torch, dict(CHORD16K, dtype='float16'), max(args.steps // 2, 1),
                                                args.warmup, args.timing_steps)
         del dd16
         dom16 = max(pk16, key=lambda k: pk16[k]['avg_ms'])

@@ -314,7 +314,7 @@ int pdd_debug_copy(pdd_dedisperser *dd, const char *name, void *host, size_t max
  * No counterpart in the reference: a small bf16 1-D convolutional network over (DM, time) tiles
  * of the dedispersed output (64 rows x 256 samples), run on MFMA, that stands beside the boxcar
  * statistic of the peak finder (PeakFindingKernel, src_lib/PeakFindingKernel.cu:252-456), which
- * stays the fallback and the judge of its completeness. Network, parameter order and rounding:
+ * stays the fallback and the baseline of its completeness. Network, parameter order and rounding:
  * pirate_amd/csrc/gpu/scorer.hpp. All pointers are device pointers; all calls are asynchronous
  * on 'stream' (a hipStream_t, NULL = default stream). */
 

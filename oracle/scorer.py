@@ -3,7 +3,7 @@
 float64 numpy restatement of the learned peak scorer (a small 1-D conv net over (DM, time)
 tiles of the dedispersed output): forward pass, mean binary cross-entropy loss and its exact
 gradient. The reference has no learned scorer; this network stands beside its boxcar peak
-statistic (src_lib/PeakFindingKernel.cu:252-456), which stays the judge of completeness.
+statistic (src_lib/PeakFindingKernel.cu:252-456), which stays the baseline of completeness.
 Only tests/ may use this module, and only as the checker.
 
 Network (one tile X: 64 DM rows x 256 samples):

@@ -1075,7 +1075,7 @@ void GpuDedisperser::release_output(long consumer, long seq_id, hipStream_t stre
 
 
 // Reference: GpuDedisperser::fill_all_weights (src_lib/GpuDedisperser.cpp:1257-1279).
-// This build's peak finder takes the per-group maximum before weighting, which equals the
+// The peak finder here takes the per-group maximum before weighting, which equals the
 // maximum of the weighted candidates only for non-negative weights; negative or non-finite
 // weights are rejected.
 void GpuDedisperser::fill_all_weights(long itree, const float *host, long nelts)

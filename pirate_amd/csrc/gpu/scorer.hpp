@@ -2,7 +2,7 @@
 // dedispersed output, run on gfx950 bf16 MFMA, with its training kernels.
 //
 // The reference has no learned scorer: this network stands beside the boxcar peak statistic
-// of src_lib/PeakFindingKernel.cu:252-456, which stays the fallback and the judge of its
+// of src_lib/PeakFindingKernel.cu:252-456, which stays the fallback and the baseline of its
 // completeness.
 //
 // Network (one tile X: 64 DM rows x 256 samples; "same" convolutions, zero outside 0..255):

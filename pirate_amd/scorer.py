@@ -4,7 +4,7 @@ dedispersed output, run and trained on gfx950 MFMA by the library's own kernels
 
 The reference has no learned scorer. The network stands beside the boxcar statistic of the
 peak finder (src_lib/PeakFindingKernel.cu:252-456): the peak finder stays the fallback and the
-judge of the scorer's completeness (completeness at equal false-alarm rate, measured by
+baseline of the scorer's completeness (completeness at equal false-alarm rate, measured by
 pirate_amd.scorer_data.completeness_at_far).
 
 Network on one tile X (64 DM rows x 256 samples), "same" convolutions with zero padding:

@@ -19,7 +19,7 @@ CASES = {
     'toy': dict(BASE, zone_nfreq=[640], zone_freq_edges=[400, 800], toplevel_tree_rank=8, time_samples_per_chunk=256,
                 beams_per_gpu=4, beams_per_batch=2, num_active_batches=2, frequency_subband_counts=[2, 2, 1],
                 primary_trees=[pt(0, 16, 16, 16), pt(1, 8, 16, 16), pt(1, 4, 16, 16)]),
-    # One primary tree, no subbands (the cfg1 structure with a shorter chunk).
+    # One primary tree, no subbands (the structure of the rank-10 1024-channel case with a shorter chunk).
     'one_tree': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
                      time_samples_per_chunk=256, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                      frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
@@ -80,10 +80,10 @@ CASES = {
                  time_samples_per_chunk=1024, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
                  frequency_subband_counts=[1], primary_trees=[pt(0, 8, 16, 16), pt(0, 8, 16, 16), pt(0, 4, 16, 16),
                                                               pt(0, 2, 16, 16), pt(0, 2, 16, 16), pt(0, 1, 16, 16)]),
-    # cfg1: 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
-    'cfg1': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
-                 time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
-                 frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
+    # 1 beam, 1024 channels 400-800 MHz, rank 10, one tree, chunks of 2048.
+    'r10_1024ch': dict(BASE, zone_nfreq=[1024], zone_freq_edges=[400, 800], toplevel_tree_rank=10,
+                       time_samples_per_chunk=2048, beams_per_gpu=1, beams_per_batch=1, num_active_batches=1,
+                       frequency_subband_counts=[1], primary_trees=[pt(0, 16, 64, 64)]),
 }
 
 # Cases whose dense CPU checks (e.g. the analytic-variance oracle) would take minutes.
@@ -93,13 +93,14 @@ LARGE = {'rank15', 'rank16', 'rank15_sb', 'rank16_sb'}
 # primary trees (1/2/4/8 ms), chunks of 2048 samples, float32. Not fixed by the workload and
 # chosen here: no subband search ([1]), max_width 16/8/4/2 (16 ms at every downsampling),
 # weight downsampling 64x64.
-CFG2 = dict(BASE, zone_nfreq=[16384], zone_freq_edges=[300, 1500], toplevel_tree_rank=14, time_samples_per_chunk=2048,
-            beams_per_gpu=16, beams_per_batch=16, num_active_batches=1, frequency_subband_counts=[1],
-            primary_trees=[pt(0, 16, 64, 64), pt(0, 8, 64, 64), pt(0, 4, 64, 64), pt(0, 2, 64, 64)])
+CHORD16K = dict(BASE, zone_nfreq=[16384], zone_freq_edges=[300, 1500], toplevel_tree_rank=14,
+                time_samples_per_chunk=2048, beams_per_gpu=16, beams_per_batch=16, num_active_batches=1,
+                frequency_subband_counts=[1],
+                primary_trees=[pt(0, 16, 64, 64), pt(0, 8, 64, 64), pt(0, 4, 64, 64), pt(0, 2, 64, 64)])
 
 
-def cfg2_with_beams(nbeams, batch=None):
-    d = dict(CFG2)
+def chord16k_with_beams(nbeams, batch=None):
+    d = dict(CHORD16K)
     d['beams_per_gpu'] = nbeams
     d['beams_per_batch'] = batch or nbeams
     return d
@@ -110,7 +111,7 @@ def cfg2_with_beams(nbeams, batch=None):
 # float16 storage, chunks of 2048 samples, run as one GpuDedisperserGroup (each group's chain on
 # a branch stream of its own per chunk). Not fixed by the workload and chosen here: no subband search ([1]), max_width 16 ms
 # at every downsampling level, weight downsampling 64x64.
-def cfg4_group(rank, nbeams, dtype='float16'):
+def batched_group(rank, nbeams, dtype='float16'):
     nfreq = {12: 4096, 13: 8192, 14: 16384}[rank]
     npri = {12: 3, 13: 4, 14: 5}[rank]
     widths = [16, 8, 4, 2, 1][:npri]
@@ -119,5 +120,5 @@ def cfg4_group(rank, nbeams, dtype='float16'):
                 frequency_subband_counts=[1], primary_trees=[pt(0, w, 64, 64) for w in widths])
 
 
-def cfg4_groups(nbeams=32, dtype='float16'):
-    return [cfg4_group(r, nbeams, dtype) for r in (12, 13, 14)]
+def batched_groups(nbeams=32, dtype='float16'):
+    return [batched_group(r, nbeams, dtype) for r in (12, 13, 14)]

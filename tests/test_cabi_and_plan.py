@@ -18,7 +18,7 @@ import pytest
 import pirate_amd as pa
 from pirate_amd import _lib
 
-from gpu_cases import CASES, CFG2
+from gpu_cases import CASES, CHORD16K
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -141,9 +141,9 @@ def test_plan_matches_reference_example_plan():
         assert t.trigger_frequency == pytest.approx(g['trigger_frequency'], rel=1e-12)
 
 
-@pytest.mark.parametrize('name', sorted(CASES) + ['cfg2'])
+@pytest.mark.parametrize('name', sorted(CASES) + ['chord16k'])
 def test_plan_trees_match_oracle(orc, name):
-    d = CFG2 if name == 'cfg2' else CASES[name]
+    d = CHORD16K if name == 'chord16k' else CASES[name]
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
     ot = orc.OracleConfig.from_yaml_dict(d).trees()
     assert len(ot) == plan.ntrees

@@ -14,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from gpu_cases import cfg2_with_beams
+from gpu_cases import chord16k_with_beams
 from gpu_harness import OracleWorker, random_weights
 
 pytestmark = pytest.mark.gpu
@@ -36,14 +36,14 @@ def longest_lag_chunks(plan):
 
 
 @pytest.mark.timeout(900)
-def test_cfg2_shapes_bit_exact_vs_oracle():
+def test_chord16k_shapes_bit_exact_vs_oracle():
     import torch
     import pirate_amd as pa
-    d = cfg2_with_beams(2)
+    d = chord16k_with_beams(2)
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
     assert plan.ntrees == 4 and plan.nfreq == 16384 and plan.nt_in == 2048
     assert longest_lag_chunks(plan) <= NCHUNKS, longest_lag_chunks(plan)
-    log(f'[gpu cfg2] 2 beams x 16384 ch, rank 14, 4 trees, T=2048, float32: {NCHUNKS} chunks vs oracle')
+    log(f'[gpu chord16k] 2 beams x 16384 ch, rank 14, 4 trees, T=2048, float32: {NCHUNKS} chunks vs oracle')
     dd = pa.GpuDedisperser(plan)
     dd.allocate()
     rng = np.random.default_rng(2)
@@ -68,7 +68,7 @@ def test_cfg2_shapes_bit_exact_vs_oracle():
                 ga = [a[b:b + 1].cpu().numpy().view(np.uint32) for a in out.out_argmax]
                 w.q.put((ichunk, x[b:b + 1], gm, ga))
         if ichunk % 8 == 0:
-            log(f'[gpu cfg2] chunk {ichunk} launched; oracle checked {[w.nchecked for w in workers]}, '
+            log(f'[gpu chord16k] chunk {ichunk} launched; oracle checked {[w.nchecked for w in workers]}, '
                 f'mismatches {sum(len(w.bad) for w in workers)}')
     for w in workers:
         w.q.put(None)
@@ -77,5 +77,5 @@ def test_cfg2_shapes_bit_exact_vs_oracle():
         assert w.err is None, w.err
         assert w.nchecked == NCHUNKS
     bad = [r for w in workers for r in w.bad]
-    log(f'[gpu cfg2] done: {NCHUNKS} chunks x 2 beams x 4 trees, {len(bad)} mismatching (chunk, tree) records')
+    log(f'[gpu chord16k] done: {NCHUNKS} chunks x 2 beams x 4 trees, {len(bad)} mismatching (chunk, tree) records')
     assert not bad, bad[:5]

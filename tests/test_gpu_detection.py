@@ -18,7 +18,7 @@ def test_injected_pulse_is_recovered():
     from pirate_amd import simpulse
     from oracle import oracle as orc
 
-    d = CASES['cfg1']                   # 1 beam, 1024 ch 400-800 MHz, rank 10, T = 2048
+    d = CASES['r10_1024ch']                   # 1 beam, 1024 ch 400-800 MHz, rank 10, T = 2048
     cfg = pa.DedispersionConfig.from_dict(d)
     plan = pa.DedispersionPlan(cfg)
     dd = pa.GpuDedisperser(plan)

@@ -10,7 +10,7 @@ stay below 8 sigma."""
 import numpy as np
 import pytest
 
-from gpu_cases import CFG2
+from gpu_cases import CHORD16K
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +20,7 @@ def test_chord16k_pulses_recovered():
     import pirate_amd as pa
     from pirate_amd import simpulse
 
-    cfg = pa.DedispersionConfig.from_dict(CFG2)
+    cfg = pa.DedispersionConfig.from_dict(CHORD16K)
     plan = pa.DedispersionPlan(cfg)
     dd = pa.GpuDedisperser(plan)
     dd.allocate()

@@ -19,7 +19,7 @@ import sys
 import numpy as np
 import pytest
 
-from gpu_cases import BASE, cfg4_groups, pt
+from gpu_cases import BASE, batched_groups, pt
 from gpu_harness import OracleWorker, random_weights
 
 pytestmark = pytest.mark.gpu
@@ -126,14 +126,14 @@ def test_group_protocol_errors_latch():
         g.members[0]._acquire_input(0, 0)
 
 
-def test_cfg4_group_bit_exact():
+def test_batched_groups_bit_exact():
     import torch
     import pirate_amd as pa
-    cfgs = cfg4_groups(nbeams=2)
+    cfgs = batched_groups(nbeams=2)
     plans = [pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d)) for d in cfgs]
     nchunks = max(max_lag_chunks(p) for p in plans)
     ncheck = [max_lag_chunks(plans[0]), max_lag_chunks(plans[1]), max_lag_chunks(plans[2], trees=(0, 1))]
-    log(f'[gpu group] cfg4 shapes, 2 beams per group, float16: {nchunks} chunks vs standalone dedispersers, '
+    log(f'[gpu group] batched shapes, 2 beams per group, float16: {nchunks} chunks vs standalone dedispersers, '
         f'oracle on the first {ncheck} chunks per group')
     g = pa.GpuDedisperserGroup(plans, use_graph=False)
     g.allocate()
@@ -178,7 +178,7 @@ def test_cfg4_group_bit_exact():
                     gm, ga = host_outputs(outs[m], b)
                     w.q.put((ichunk, xs[m][b:b + 1].to(torch.float32).cpu().numpy(), gm, ga))
         if ichunk % 16 == 0:
-            log(f'[gpu group] cfg4 chunk {ichunk}: {len(mism)} mismatches vs standalone so far')
+            log(f'[gpu group] batched chunk {ichunk}: {len(mism)} mismatches vs standalone so far')
     for _, _, w in workers:
         w.q.put(None)
     for m, b, w in workers:

@@ -17,7 +17,7 @@ from gpu_harness import run_and_compare
 pytestmark = pytest.mark.gpu
 
 NCHUNKS = {'toy': 4, 'one_tree': 3, 'zones4': 3, 'subbands': 3, 'wide_ch': 3, 'half_tile': 5, 'short_chunk': 12,
-           'rank15': 9, 'rank16': 3, 'rank15_sb': 4, 'rank16_sb': 2, 'cfg1': 4, 'pri5': 5, 'pri6': 7}
+           'rank15': 9, 'rank16': 3, 'rank15_sb': 4, 'rank16_sb': 2, 'r10_1024ch': 4, 'pri5': 5, 'pri6': 7}
 
 
 def log(msg):
@@ -26,14 +26,14 @@ def log(msg):
 
 
 @pytest.mark.parametrize('name', ['toy', 'one_tree', 'zones4', 'subbands', 'wide_ch', 'half_tile', 'short_chunk', 'rank15',
-                                  'rank16', 'rank15_sb', 'rank16_sb', 'cfg1', 'pri5', 'pri6'])
+                                  'rank16', 'rank15_sb', 'rank16_sb', 'r10_1024ch', 'pri5', 'pri6'])
 def test_gpu_bit_exact_vs_oracle(name):
     log(f'[gpu parity] case={name} nchunks={NCHUNKS[name]}')
     bad, dd, plan = run_and_compare(CASES[name], NCHUNKS[name], seed=11, log=log)
     assert not bad, f'{len(bad)} mismatching (seq, tree) outputs, first: {bad[0]}'
 
 
-def test_gpu_gaussian_input_cfg1():
-    log('[gpu parity] case=cfg1 gaussian input')
-    bad, dd, plan = run_and_compare(CASES['cfg1'], 2, seed=5, input_kind='normal', log=log)
+def test_gpu_gaussian_input_r10_1024ch():
+    log('[gpu parity] case=r10_1024ch gaussian input')
+    bad, dd, plan = run_and_compare(CASES['r10_1024ch'], 2, seed=5, input_kind='normal', log=log)
     assert not bad, f'first mismatch: {bad[0]}'

@@ -39,9 +39,9 @@ def _check(stats, dtype, check_max=True):
 
 
 @pytest.mark.parametrize('dtype', ['float16', pytest.param('bfloat16', marks=BF16_MAX)])
-@pytest.mark.parametrize('name', ['cfg1', 'toy', 'zones4'])
+@pytest.mark.parametrize('name', ['r10_1024ch', 'toy', 'zones4'])
 def test_reduced_precision_vs_fp64_oracle(name, dtype):
-    nchunks = 2 if name == 'cfg1' else 4
+    nchunks = 2 if name == 'r10_1024ch' else 4
     _check(run_precision(CASES[name], dtype, nchunks, seed=7), dtype)
 
 
@@ -49,17 +49,17 @@ def test_reduced_precision_vs_fp64_oracle(name, dtype):
 def test_reduced_precision_with_rfi_vs_fp64_oracle(dtype):
     """The 1024-channel rank-10 configuration with narrowband RFI (1% of the channels at +5
     sigma), 3 chunks."""
-    _check(run_precision(CASES['cfg1'], dtype, 3, seed=8, rfi=True), dtype)
+    _check(run_precision(CASES['r10_1024ch'], dtype, 3, seed=8, rfi=True), dtype)
 
 
 def test_bfloat16_rms_vs_fp64_oracle():
     """bfloat16: the rms error bound (1e-2) holds, with and without RFI."""
-    _check(run_precision(CASES['cfg1'], 'bfloat16', 2, seed=7), 'bfloat16', check_max=False)
-    _check(run_precision(CASES['cfg1'], 'bfloat16', 2, seed=8, rfi=True), 'bfloat16', check_max=False)
+    _check(run_precision(CASES['r10_1024ch'], 'bfloat16', 2, seed=7), 'bfloat16', check_max=False)
+    _check(run_precision(CASES['r10_1024ch'], 'bfloat16', 2, seed=8, rfi=True), 'bfloat16', check_max=False)
 
 
 @pytest.mark.parametrize('dtype', ['float16', 'bfloat16'])
-@pytest.mark.parametrize('name', ['toy', 'zones4', 'subbands', 'cfg1', 'pri5'])
+@pytest.mark.parametrize('name', ['toy', 'zones4', 'subbands', 'r10_1024ch', 'pri5'])
 def test_16bit_storage_bit_exact_vs_rounding_oracle(name, dtype):
     """With 16-bit storage the GPU rounds to the storage type only where it stores (gridded
     rows, downsampled rows, lag ring, subband rows, out_max) and computes in float32 in

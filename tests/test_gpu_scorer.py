@@ -9,7 +9,7 @@ with rounding='bf16': bf16 at exactly the places the kernels hold bf16 values).
   * bit-identical gradients on repeated calls (fixed-order reduction);
   * tiles read in place from strided float32 / float16 rows with origins (and an out-of-range
     origin -> NaN) against the same tiles gathered into a contiguous array;
-  * tiles read from a dedisperser's tree rows (cfg1 shapes) against the rows copied to the host.
+  * tiles read from a dedisperser's tree rows (rank-10, 1024-channel shapes) against the rows copied to the host.
 
 Tolerances: the kernels accumulate in float32 and the oracle in float64; a float32 rounding can
 move a value across a bf16 rounding boundary (one bf16 ulp, 2^-8 relative, on rare elements), so
@@ -152,7 +152,7 @@ def test_tree_rows_of_a_dedisperser():
     import torch
     import pirate_amd as pa
     from gpu_cases import CASES
-    d = dict(CASES['cfg1'], beams_per_gpu=2, beams_per_batch=2)
+    d = dict(CASES['r10_1024ch'], beams_per_gpu=2, beams_per_batch=2)
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
     dd = pa.GpuDedisperser(plan)
     dd.allocate()
@@ -187,7 +187,7 @@ def test_scorer_in_chain(dtype):
     import torch
     import pirate_amd as pa
     from gpu_cases import CASES
-    d = dict(CASES['cfg1'], beams_per_gpu=4, beams_per_batch=2, dtype=dtype)
+    d = dict(CASES['r10_1024ch'], beams_per_gpu=4, beams_per_batch=2, dtype=dtype)
     plan = pa.DedispersionPlan(pa.DedispersionConfig.from_dict(d))
     dd = pa.GpuDedisperser(plan)
     dd.allocate()

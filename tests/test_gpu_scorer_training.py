@@ -1,4 +1,4 @@
-"""The learned scorer trained on the GPU by its own kernels, and judged by the plain peak finder.
+"""The learned scorer trained on the GPU by its own kernels, and measured against the plain peak finder.
 
 Data (pirate_amd.scorer_data): 64 x 256 tiles of the 1024-channel rank-10 chain's dedispersed
 output (400-800 MHz, 1 ms, chunks of 2048, 16 beams per chunk), with simulated pulses
