@@ -10,7 +10,7 @@
 //                    the even row, then the "large" lag L_j = 2^(s-1) - 1 - (j mod 2^(s-1)))
 //
 // Mapping: one wave per (tree-channel pair (2j, 2j+1), time tile of 64 x G samples),
-// G = max(4 (16 for 16-bit storage), 2^(npri-1)); a workgroup holds 4 consecutive
+// G = max(16, 2^(npri-1)); a workgroup holds 4 consecutive
 // (pair, tile) waves in pair-major order (4 tiles of one pair, or 2 whole pairs when a pair
 // has 2 tiles). The wave grids its tile in a coalesced layout (lane l: 16 bytes of
 // consecutive samples in each block of 64 x 16 bytes, so every load and store instruction
@@ -55,15 +55,15 @@ long lds_state_nelts(int R, int npri, int s)
     return 2 * (npri - 1) * nrows * (1 + lcap);     // two parities
 }
 
-// Samples per lane in the downsampling layout: at least 4, and 16 for 16-bit storage, so that
-// the gridding loads and stores are 16 bytes per lane and each wave moves as many bytes as a
-// float32 wave (with 8, the per-wave work of the transposes, the hand-over and the
-// downsampling levels was spread over half the bytes: 0.69 -> 0.63 ms for 16 beams x 16384
-// channels with 4 primary trees). The chunk length is a multiple of 32 samples, so 16 divides it.
+// Samples per lane in the downsampling layout: at least 16, so that each wave's fixed work
+// (the pair's set-up, the transposes, the hand-over and the downsampling levels) is spread
+// over 64 or 32 bytes of each row per lane (16 beams x 16384 channels with 4 primary trees:
+// float16 0.69 -> 0.63 ms and float32 1.07 -> 1.03 ms against 8 samples per lane). The chunk
+// length is a multiple of 32 samples, so 16 divides it.
 template<typename Tst>
 __host__ __device__ constexpr int grid_lds_min_group()
 {
-    return sizeof(Tst) == 2 ? 16 : 4;
+    return 16;
 }
 
 // Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/V groups of V consecutive
