@@ -60,11 +60,7 @@ long lds_state_nelts(int R, int npri, int s)
 // over 64 or 32 bytes of each row per lane (16 beams x 16384 channels with 4 primary trees:
 // float16 0.69 -> 0.63 ms and float32 1.07 -> 1.03 ms against 8 samples per lane). The chunk
 // length is a multiple of 32 samples, so 16 divides it.
-template<typename Tst>
-__host__ __device__ constexpr int grid_lds_min_group()
-{
-    return 16;
-}
+static constexpr int kGlMinGroup = 16;
 
 // Grids tree channels 2j ("e") and 2j+1 ("o") for G samples, G/V groups of V consecutive
 // samples at in + QS*q (q < nq; groups q >= nq are outside the chunk and left at zero). The
@@ -122,7 +118,7 @@ template<typename Tst, int NPRI>
 __global__ void __launch_bounds__(kGlWaves * 64)
 k_grid_lds(GridLdsArgs a)
 {
-    constexpr int G = std::max(grid_lds_min_group<Tst>(), 1 << (NPRI - 1));
+    constexpr int G = std::max(kGlMinGroup, 1 << (NPRI - 1));
     constexpr int NL = (NPRI > 1) ? NPRI : 2;          // array size for per-level values
 
     const int lane = threadIdx.x & 63;
@@ -160,11 +156,10 @@ k_grid_lds(GridLdsArgs a)
     const int t0 = g * G;
 
     // Gridding in the coalesced layout: lane l holds samples tb0 + 64 V q + V l + (0..V-1),
-    // q < G/V, V = 16 bytes of samples where that divides G (else 4), so that each load and
-    // store instruction of the wave covers 64 V consecutive samples of a row. With V = G this
-    // is already G consecutive samples per lane. Groups past the chunk end (a partial last
-    // tile) are skipped.
-    constexpr int V = (sizeof(Tst) == 2 && G % 8 == 0) ? 8 : 4;
+    // q < G/V, V = 16 bytes of samples (4 float32 or 8 16-bit values; V divides G), so that
+    // each load and store instruction of the wave covers 64 V consecutive samples of a row.
+    // Groups past the chunk end (a partial last tile) are skipped.
+    constexpr int V = 16 / int(sizeof(Tst));
     constexpr int BLK = 64 * V;
     const int tb0 = g0 * G;
     const int nq = min(G / V, max(0, (a.ntime - tb0 - V * lane + BLK - 1) / BLK));
@@ -346,7 +341,7 @@ k_grid_lds(GridLdsArgs a)
 template<typename Tst>
 static void launch_grid_lds_t(const GridLdsArgs &a, hipStream_t st)
 {
-    const int G = std::max(grid_lds_min_group<Tst>(), 1 << (a.npri - 1));
+    const int G = std::max(kGlMinGroup, 1 << (a.npri - 1));
     const long nwaves = long(a.nchan >> 1) * ((a.ntime / G + 63) / 64);
     dim3 grid(unsigned((nwaves + kGlWaves - 1) / kGlWaves), a.B);   // 4 independent (pair, tile) waves each
     dim3 block(kGlWaves * 64);
