@@ -17,7 +17,7 @@ from random_configs import draw_supported
 
 pytestmark = pytest.mark.gpu
 
-NSEEDS = 100
+NSEEDS = 300
 
 
 def log(msg):
