@@ -35,7 +35,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('dir')
     ap.add_argument('--md', required=True)
-    ap.add_argument('--title', required=True)
+    ap.add_argument('--title', default='rocprofv3 kernel statistics')
     ap.add_argument('--command', action='append', default=[])
     ap.add_argument('--note', action='append', default=[])
     ap.add_argument('--flop', action='append', default=[])
